@@ -1,0 +1,1535 @@
+// Host orchestration and C ABI of the afx FDTD core.
+//
+// Reference counterparts (paths relative to the reference root):
+//   afx_sim                 Simulation, anc-field/src/simulation.hpp:83-156
+//   classify_room           LrsData<T>::Init / GetGroupIx, anc-field/src/lrs-data.hpp:117-330
+//   run_chunk               Simulation::Run step loop, simulation.hpp:342-434
+//   afx_set_pml (compat)    Simulation::SetPml, simulation.hpp:233-277
+//   afx_add_source/receiver simulation.hpp:309-340
+//   afx_set_min_size        simulation.hpp:279-284
+#include "afx_internal.hpp"
+#include "../../include/afx.h"
+
+#include <algorithm>
+#include <map>
+#include <tuple>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace afx {
+int load_kerkythea(const char* path, float res, int32_t dims[3], std::vector<int32_t>& vox, std::vector<float>& a,
+                   std::vector<float>& b, int32_t* ncoef, int32_t* nmat, std::string& err);
+}
+
+using namespace afx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) return fail(AFX_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// Reference rounding of grid dims, util.hpp:43-47
+uint32_t round_to(uint32_t n, uint32_t unit) { return n % unit == 0 ? n : n + (unit - n % unit); }
+
+size_t prec_bytes(Prec p)
+{
+    switch (p) {
+    case Prec::F32: return 4;
+    case Prec::F64: return 8;
+    case Prec::F16: return 2;
+    case Prec::BF16: return 2;
+    }
+    return 4;
+}
+
+// Device buffer with RAII release.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        release();
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Room {
+    int32_t nx = 0, ny = 0, nz = 0;
+    std::vector<int32_t> vox;  // empty: all air
+    int32_t vx = 0, vy = 0, vz = 0;  // voxel volume dims (<= grid)
+    int32_t mat_base = 0, nmat = 0;
+    int64_t base = 0;
+};
+
+struct Source {
+    int32_t room;
+    int32_t cell[3];
+    std::vector<float> samples;
+};
+
+struct Mic {
+    int32_t room;
+    int32_t cell[3];
+};
+
+struct Entry {  // host-side boundary entry
+    int64_t cell;   // global linear index
+    uint32_t info;
+    uint32_t mat;
+    int32_t room, x, y, z;
+};
+
+const uint32_t kGroupMask[26] = {
+    1, 2, 4, 8, 16, 32,
+    1 | 4, 1 | 8, 1 | 16, 1 | 32, 2 | 4, 2 | 8, 2 | 16, 2 | 32, 4 | 16, 4 | 32, 8 | 16, 8 | 32,
+    1 | 4 | 16, 1 | 4 | 32, 1 | 8 | 16, 1 | 8 | 32, 2 | 4 | 16, 2 | 4 | 32, 2 | 8 | 16, 2 | 8 | 32,
+};
+
+// mask (bit order X_P, X_M, Y_P, Y_M, Z_P, Z_M) -> reference group id, -1 if unphysical
+int group_of_mask(uint32_t m)
+{
+    for (int g = 0; g < 26; ++g)
+        if (kGroupMask[g] == m) return g;
+    return -1;
+}
+
+}  // namespace
+
+struct afx_sim {
+    afx_config cfg{};
+    Prec prec = Prec::F32;
+    bool compat = true;
+    hipStream_t stream = nullptr;
+    bool has_model = false;  // LRS active (simulation.hpp:363)
+    // room 0 grid control
+    bool min_size_set = false;
+    int32_t min_dims[3] = {0, 0, 0};
+    std::vector<Room> rooms;
+    int32_t ncoef = 0;
+    std::vector<float> A, B;  // [nmat_total][ncoef]
+    // pml
+    int pml_mode = 0;  // 0 none, 1 compat, 2 physical
+    int32_t pml_width = 0;
+    uint32_t pml_faces = 0;
+    std::vector<float> pml_profile;
+    std::vector<Source> srcs;
+    std::vector<Mic> mics;
+    int tile = 0;
+    bool fused_ok = true;
+
+    // ---- device state
+    bool committed = false;
+    bool dirty_special = false;
+    int64_t total_cells = 0;
+    DBuf P[3];
+    int rot[3] = {0, 1, 2};
+    DBuf d_rooms, d_tiles, d_tile_eoff, d_tile_eperm, d_tile_spoff;
+    int32_t ntiles = 0;
+    DBuf d_e_cell, d_e_info, d_e_mat, d_ring_in, d_ring_out, d_A, d_B;
+    int32_t nb = 0;
+    std::vector<Entry> entries;
+    DBuf d_sp_cell, d_sp_room, d_sp_bidx, d_sp_src_off, d_sp_src, d_sp_mic_off, d_sp_mic;
+    int32_t nsp = 0;
+    DBuf d_src, d_src_len;
+    int64_t src_cap = 0;
+    DBuf d_traces;
+    int64_t trace_cap = 0;
+    DBuf d_t;
+    int64_t t = 0;  // host mirror of the step counter
+    double scale = 1.0;
+    // pml device state
+    DBuf d_phi[3], d_phi_old[3], d_psi, d_psi_old, d_z[3], d_zh[3], d_boxes, d_axis_off;
+    int32_t nbox = 0;
+    int64_t box_cells_max = 0;
+    int pml_rot = 0;
+    // graph
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    int graph_steps = 30;
+    int64_t group_count[26] = {0};
+    double last_run_ms = 0.0;
+
+    ~afx_sim()
+    {
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graphs()
+    {
+        for (auto& g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+};
+
+namespace {
+
+int check_sim(afx_sim* s)
+{
+    if (!s) return fail(AFX_E_ARG, "null handle");
+    return AFX_OK;
+}
+
+// ---- geometry ---------------------------------------------------------------
+
+// Classify the air cells of one room (lrs-data.hpp:117-215): cells with
+// 1 <= x <= nx-2 (likewise y, z) whose 6-neighbourhood holds solid voxels.
+int classify_room(afx_sim* s, int ridx, std::vector<Entry>& out)
+{
+    const Room& r = s->rooms[ridx];
+    if (r.vox.empty()) return AFX_OK;
+    const int64_t sx = r.nx, sxy = (int64_t)r.nx * r.ny;
+    auto V = [&](int64_t x, int64_t y, int64_t z) -> int32_t {
+        if (x >= r.vx || y >= r.vy || z >= r.vz) return -1;  // air padding beyond the voxel volume
+        return r.vox[(size_t)((z * r.vy + y) * r.vx + x)];
+    };
+    for (int64_t z = 1; z < r.nz - 1; ++z)
+        for (int64_t y = 1; y < r.ny - 1; ++y)
+            for (int64_t x = 1; x < r.nx - 1; ++x) {
+                if (V(x, y, z) != -1) continue;
+                const int32_t nb6[6] = {V(x + 1, y, z), V(x - 1, y, z), V(x, y + 1, z),
+                                        V(x, y - 1, z), V(x, y, z + 1), V(x, y, z - 1)};
+                uint32_t m = 0;
+                for (int f = 0; f < 6; ++f)
+                    if (nb6[f] != -1) m |= 1u << f;
+                if (!m) continue;
+                const int g = group_of_mask(m);
+                if (g < 0)
+                    return fail(AFX_E_GEOMETRY, "Unphysical surfaces encountered at room %d cell (%lld,%lld,%lld)",
+                                ridx, (long long)x, (long long)y, (long long)z);
+                const int K = g <= 5 ? 1 : (g <= 17 ? 2 : 3);
+                int32_t mats[3] = {0, 0, 0};
+                int k = 0;
+                for (int f = 0; f < 6; ++f)
+                    if (nb6[f] != -1) mats[k++] = nb6[f];
+                uint32_t mw = 0;
+                for (int q = 0; q < K; ++q) {
+                    if (mats[q] < 0 || mats[q] >= r.nmat)
+                        return fail(AFX_E_ARG, "room %d: voxel material %d outside 0..%d", ridx, mats[q], r.nmat - 1);
+                    mw |= (uint32_t)(mats[q] + r.mat_base) << (10 * q);
+                }
+                Entry e;
+                e.cell = r.base + z * sxy + y * sx + x;
+                e.info = m | ((uint32_t)g << kInfoGroupShift) | ((uint32_t)K << kInfoKShift) |
+                         ((uint32_t)ridx << kInfoRoomShift);
+                if (s->compat && g == 25) e.info |= kInfoNoUpdate;
+                e.mat = mw;
+                e.room = ridx;
+                e.x = (int32_t)x;
+                e.y = (int32_t)y;
+                e.z = (int32_t)z;
+                out.push_back(e);
+                s->group_count[g]++;
+            }
+    return AFX_OK;
+}
+
+template <typename T>
+hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t st)
+{
+    hipError_t e = d.alloc(std::max<size_t>(h.size() * sizeof(T), 256));
+    if (e != hipSuccess) return e;
+    if (!h.empty()) e = hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st);
+    return e;
+}
+
+// Tile table in XCD-swizzled order: workgroup i runs on XCD i % 8, so tile
+// slot i takes logical tile (i % 8) * per + i / 8 and each XCD walks a
+// contiguous run of tiles whose halos it shares in its own L2.
+void build_tiles(afx_sim* s, std::vector<int4>& table, std::vector<int32_t>& logical_of_slot)
+{
+    const TileShape ts = kTiles[s->tile];
+    std::vector<int4> logical;
+    for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+        const Room& r = s->rooms[ri];
+        for (int z0 = 0; z0 < r.nz; z0 += ts.tz)
+            for (int y0 = 0; y0 < r.ny; y0 += ts.ty)
+                for (int x0 = 0; x0 < r.nx; x0 += ts.tx) logical.push_back(make_int4((int)ri, x0, y0, z0));
+    }
+    const int64_t n = (int64_t)logical.size();
+    const int64_t per = (n + 7) / 8;
+    table.assign((size_t)(per * 8), make_int4(-1, 0, 0, 0));
+    logical_of_slot.assign((size_t)(per * 8), -1);
+    for (int64_t i = 0; i < per * 8; ++i) {
+        const int64_t L = (i % 8) * per + i / 8;
+        if (L < n) {
+            table[(size_t)i] = logical[(size_t)L];
+            logical_of_slot[(size_t)i] = (int32_t)L;
+        }
+    }
+}
+
+// slot of the tile holding (room, x, y, z)
+struct TileIndex {
+    std::vector<int64_t> room_first;  // logical index of the room's first tile
+    std::vector<int32_t> slot_of_logical;
+    TileShape ts;
+    const afx_sim* s;
+    int64_t slot(int room, int x, int y, int z) const
+    {
+        const Room& r = s->rooms[room];
+        const int ntx = (r.nx + ts.tx - 1) / ts.tx, nty = (r.ny + ts.ty - 1) / ts.ty;
+        const int64_t L = room_first[room] + ((int64_t)(z / ts.tz) * nty + y / ts.ty) * ntx + x / ts.tx;
+        return slot_of_logical[(size_t)L];
+    }
+};
+
+int build_tile_tables(afx_sim* s)
+{
+    std::vector<int4> table;
+    std::vector<int32_t> logical_of_slot;
+    build_tiles(s, table, logical_of_slot);
+    s->ntiles = (int32_t)table.size();
+    TileIndex ti;
+    ti.s = s;
+    ti.ts = kTiles[s->tile];
+    int64_t acc = 0;
+    for (const Room& r : s->rooms) {
+        ti.room_first.push_back(acc);
+        acc += (int64_t)((r.nx + ti.ts.tx - 1) / ti.ts.tx) * ((r.ny + ti.ts.ty - 1) / ti.ts.ty) *
+               ((r.nz + ti.ts.tz - 1) / ti.ts.tz);
+    }
+    ti.slot_of_logical.assign((size_t)acc, -1);
+    for (size_t i = 0; i < logical_of_slot.size(); ++i)
+        if (logical_of_slot[i] >= 0) ti.slot_of_logical[(size_t)logical_of_slot[i]] = (int32_t)i;
+    // boundary entries per tile slot (indices into the linear-order entry arrays)
+    std::vector<int32_t> cnt((size_t)s->ntiles + 1, 0), eoff((size_t)s->ntiles + 1, 0), perm((size_t)s->nb);
+    std::vector<int64_t> eslot((size_t)s->nb);
+    for (int32_t e = 0; e < s->nb; ++e) {
+        const Entry& en = s->entries[(size_t)e];
+        eslot[(size_t)e] = ti.slot(en.room, en.x, en.y, en.z);
+        cnt[(size_t)eslot[(size_t)e]]++;
+    }
+    for (int32_t i = 0; i < s->ntiles; ++i) eoff[(size_t)i + 1] = eoff[(size_t)i] + cnt[(size_t)i];
+    {
+        std::vector<int32_t> fill(eoff.begin(), eoff.end() - 1);
+        for (int32_t e = 0; e < s->nb; ++e) perm[(size_t)fill[(size_t)eslot[(size_t)e]]++] = e;
+    }
+    HIP_TRY(upload(s->d_tiles, table, s->stream));
+    HIP_TRY(upload(s->d_tile_eoff, eoff, s->stream));
+    HIP_TRY(upload(s->d_tile_eperm, perm, s->stream));
+    return AFX_OK;
+}
+
+// Cells holding sources or receivers.  Each is finished by one thread that
+// computes the step value, adds the sources in order and records receivers.
+int build_special(afx_sim* s)
+{
+    struct Sp {
+        int64_t cell;
+        int32_t room, x, y, z;
+        std::vector<int32_t> src, mic;
+    };
+    std::vector<Sp> sp;
+    auto find_or_add = [&](int32_t room, const int32_t c[3]) -> Sp& {
+        const Room& r = s->rooms[room];
+        const int64_t cell = r.base + ((int64_t)c[2] * r.ny + c[1]) * r.nx + c[0];
+        for (auto& q : sp)
+            if (q.cell == cell) return q;
+        Sp q;
+        q.cell = cell;
+        q.room = room;
+        q.x = c[0];
+        q.y = c[1];
+        q.z = c[2];
+        sp.push_back(q);
+        return sp.back();
+    };
+    for (size_t i = 0; i < s->srcs.size(); ++i) {
+        const Source& q = s->srcs[i];
+        const Room& r = s->rooms[q.room];
+        if (q.cell[0] < 0 || q.cell[1] < 0 || q.cell[2] < 0 || q.cell[0] >= r.nx || q.cell[1] >= r.ny || q.cell[2] >= r.nz)
+            return fail(AFX_E_RANGE, "source %zu at cell (%d,%d,%d) outside the %dx%dx%d grid", i, q.cell[0], q.cell[1],
+                        q.cell[2], r.nx, r.ny, r.nz);
+        find_or_add(q.room, q.cell).src.push_back((int32_t)i);
+    }
+    for (size_t i = 0; i < s->mics.size(); ++i) {
+        const Mic& q = s->mics[i];
+        const Room& r = s->rooms[q.room];
+        if (q.cell[0] < 0 || q.cell[1] < 0 || q.cell[2] < 0 || q.cell[0] >= r.nx || q.cell[1] >= r.ny || q.cell[2] >= r.nz)
+            return fail(AFX_E_RANGE, "receiver %zu at cell (%d,%d,%d) outside the %dx%dx%d grid", i, q.cell[0], q.cell[1],
+                        q.cell[2], r.nx, r.ny, r.nz);
+        find_or_add(q.room, q.cell).mic.push_back((int32_t)i);
+    }
+    // boundary index lookup (entries are sorted by cell)
+    std::vector<uint32_t> info((size_t)s->nb);
+    for (int32_t e = 0; e < s->nb; ++e) info[(size_t)e] = s->entries[(size_t)e].info & ~kInfoSpecial;
+    std::vector<int32_t> cell, room, bidx, soff(1, 0), sids, moff(1, 0), mids;
+    // order specials by tile slot for the fused kernel
+    TileIndex ti;
+    ti.s = s;
+    ti.ts = kTiles[s->tile];
+    std::vector<int4> table;
+    std::vector<int32_t> logical_of_slot;
+    build_tiles(s, table, logical_of_slot);
+    int64_t acc = 0;
+    for (const Room& r : s->rooms) {
+        ti.room_first.push_back(acc);
+        acc += (int64_t)((r.nx + ti.ts.tx - 1) / ti.ts.tx) * ((r.ny + ti.ts.ty - 1) / ti.ts.ty) *
+               ((r.nz + ti.ts.tz - 1) / ti.ts.tz);
+    }
+    ti.slot_of_logical.assign((size_t)acc, -1);
+    for (size_t i = 0; i < logical_of_slot.size(); ++i)
+        if (logical_of_slot[i] >= 0) ti.slot_of_logical[(size_t)logical_of_slot[i]] = (int32_t)i;
+    std::vector<std::pair<int64_t, size_t>> order;
+    for (size_t k = 0; k < sp.size(); ++k) order.push_back({ti.slot(sp[k].room, sp[k].x, sp[k].y, sp[k].z), k});
+    std::sort(order.begin(), order.end());
+    std::vector<int32_t> tsoff((size_t)s->ntiles + 1, 0);
+    for (auto& o : order) tsoff[(size_t)o.first + 1]++;
+    for (int32_t i = 0; i < s->ntiles; ++i) tsoff[(size_t)i + 1] += tsoff[(size_t)i];
+    for (auto& o : order) {
+        const Sp& q = sp[o.second];
+        const Room& r = s->rooms[q.room];
+        cell.push_back((int32_t)q.cell);
+        room.push_back(q.room);
+        int32_t b = -1;
+        const bool edge = q.x == 0 || q.y == 0 || q.z == 0 || q.x == r.nx - 1 || q.y == r.ny - 1 || q.z == r.nz - 1;
+        if (edge) {
+            b = -2;
+        } else {
+            auto it = std::lower_bound(s->entries.begin(), s->entries.end(), q.cell,
+                                       [](const Entry& e, int64_t c) { return e.cell < c; });
+            if (it != s->entries.end() && it->cell == q.cell) {
+                b = (int32_t)(it - s->entries.begin());
+                info[(size_t)b] |= kInfoSpecial;
+            }
+        }
+        bidx.push_back(b);
+        for (int32_t v : q.src) sids.push_back(v);
+        soff.push_back((int32_t)sids.size());
+        for (int32_t v : q.mic) mids.push_back(v);
+        moff.push_back((int32_t)mids.size());
+    }
+    s->nsp = (int32_t)cell.size();
+    HIP_TRY(upload(s->d_sp_cell, cell, s->stream));
+    HIP_TRY(upload(s->d_sp_room, room, s->stream));
+    HIP_TRY(upload(s->d_sp_bidx, bidx, s->stream));
+    HIP_TRY(upload(s->d_sp_src_off, soff, s->stream));
+    HIP_TRY(upload(s->d_sp_src, sids, s->stream));
+    HIP_TRY(upload(s->d_sp_mic_off, moff, s->stream));
+    HIP_TRY(upload(s->d_sp_mic, mids, s->stream));
+    HIP_TRY(upload(s->d_e_info, info, s->stream));
+    HIP_TRY(upload(s->d_tile_spoff, tsoff, s->stream));
+    // sources
+    int64_t cap = 1;
+    for (auto& q : s->srcs) cap = std::max<int64_t>(cap, (int64_t)q.samples.size());
+    std::vector<float> sbuf((size_t)(cap * std::max<size_t>(s->srcs.size(), 1)), 0.0f);
+    std::vector<int64_t> slen(std::max<size_t>(s->srcs.size(), 1), 0);
+    for (size_t i = 0; i < s->srcs.size(); ++i) {
+        std::copy(s->srcs[i].samples.begin(), s->srcs[i].samples.end(), sbuf.begin() + (int64_t)i * cap);
+        slen[i] = (int64_t)s->srcs[i].samples.size();
+    }
+    s->src_cap = cap;
+    HIP_TRY(upload(s->d_src, sbuf, s->stream));
+    HIP_TRY(upload(s->d_src_len, slen, s->stream));
+    // FP16/BF16 storage scale: a power of two at the sources' peak amplitude
+    if (s->prec == Prec::F16 || s->prec == Prec::BF16) {
+        if (s->cfg.scale > 0) {
+            s->scale = s->cfg.scale;
+        } else {
+            float mx = 0.0f;
+            for (auto& q : s->srcs)
+                for (float v : q.samples) mx = std::max(mx, std::fabs(v));
+            s->scale = mx > 0 ? std::ldexp(1.0, (int)std::ceil(std::log2((double)mx))) : 1.0;
+        }
+    } else {
+        s->scale = 1.0;
+    }
+    s->dirty_special = false;
+    s->drop_graphs();
+    return AFX_OK;
+}
+
+// Disjoint boxes covering the union of the PML slabs: each slab minus the
+// boxes already placed.
+void subtract_box(const std::array<int, 6>& a, const std::array<int, 6>& b, std::vector<std::array<int, 6>>& out)
+{
+    // a minus b, boxes half-open [x0,x1) [y0,y1) [z0,z1)
+    if (a[0] >= b[1] || a[1] <= b[0] || a[2] >= b[3] || a[3] <= b[2] || a[4] >= b[5] || a[5] <= b[4]) {
+        out.push_back(a);
+        return;
+    }
+    std::array<int, 6> r = a;
+    if (r[0] < b[0]) { auto c = r; c[1] = b[0]; out.push_back(c); r[0] = b[0]; }
+    if (r[1] > b[1]) { auto c = r; c[0] = b[1]; out.push_back(c); r[1] = b[1]; }
+    if (r[2] < b[2]) { auto c = r; c[3] = b[2]; out.push_back(c); r[2] = b[2]; }
+    if (r[3] > b[3]) { auto c = r; c[2] = b[3]; out.push_back(c); r[3] = b[3]; }
+    if (r[4] < b[4]) { auto c = r; c[5] = b[4]; out.push_back(c); r[4] = b[4]; }
+    if (r[5] > b[5]) { auto c = r; c[4] = b[5]; out.push_back(c); r[5] = b[5]; }
+}
+
+template <typename C>
+int setup_pml_t(afx_sim* s)
+{
+    const Room& r = s->rooms[0];
+    const int dims[3] = {r.nx, r.ny, r.nz};
+    const size_t cbytes = sizeof(C) * (size_t)s->total_cells;
+    if (s->pml_mode == 1) {
+        // SetPml: profile reversed into both ends of each axis (simulation.hpp:268-276)
+        const int dmax = std::max(r.nx, std::max(r.ny, r.nz)) + 2;
+        for (int d = 0; d < 3; ++d) {
+            std::vector<C> k((size_t)dmax, (C)0);
+            const int len = (int)s->pml_profile.size();
+            for (int i = 0; i < len && i < dims[d]; ++i) {
+                k[(size_t)i] = (C)s->pml_profile[(size_t)(len - 1 - i)];
+                k[(size_t)(dims[d] - i - 1)] = (C)s->pml_profile[(size_t)(len - 1 - i)];
+            }
+            HIP_TRY(upload(s->d_z[d], k, s->stream));
+            HIP_TRY(s->d_phi[d].alloc(cbytes));
+            HIP_TRY(s->d_phi_old[d].alloc(cbytes));
+            HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)cbytes, s->stream));
+            HIP_TRY(launch_fill(s->d_phi_old[d].p, (int64_t)cbytes, s->stream));
+        }
+        HIP_TRY(s->d_psi.alloc(cbytes));
+        HIP_TRY(s->d_psi_old.alloc(cbytes));
+        HIP_TRY(launch_fill(s->d_psi.p, (int64_t)cbytes, s->stream));
+        HIP_TRY(launch_fill(s->d_psi_old.p, (int64_t)cbytes, s->stream));
+        s->pml_rot = 0;
+        return AFX_OK;
+    }
+    // physical: graded zeta on the selected faces of room 0
+    const int w = s->pml_width;
+    const double order = s->cfg.pml_order > 0 ? s->cfg.pml_order : 3.0;
+    const double r0 = s->cfg.pml_r0 > 0 ? s->cfg.pml_r0 : 1e-3;
+    const double zmax = (order + 1.0) * s->cfg.c * std::log(1.0 / r0) / (2.0 * w * s->cfg.dx);
+    auto prof = [&](double depth) -> double {  // depth in cells from the interface
+        if (depth <= 0) return 0.0;
+        if (!s->pml_profile.empty()) {
+            // user values at cell-centre depths 0.5 .. w-0.5, linear in between
+            const double u = depth - 0.5;
+            const int lo = (int)std::floor(u);
+            const double f = u - lo;
+            auto at = [&](int k) { return k < 0 ? 0.0 : (k >= w ? (double)s->pml_profile[(size_t)w - 1] : (double)s->pml_profile[(size_t)k]); };
+            return (1.0 - f) * at(lo) + f * at(lo + 1);
+        }
+        return zmax * std::pow(std::min(depth, (double)w) / w, order);
+    };
+    std::vector<std::array<int, 6>> boxes;
+    for (int d = 0; d < 3; ++d) {
+        std::vector<C> zc((size_t)dims[d] + 1, (C)0), zh((size_t)dims[d] + 1, (C)0);
+        const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
+        const int n = dims[d];
+        for (int i = 0; i < n; ++i) {
+            double vc = 0, vh = 0;
+            if (hi) {  // interface at n-1-w-0.5
+                vc = std::max(vc, prof(i - (n - 1 - w) + 0.5));
+                vh = std::max(vh, prof(i - (n - 1 - w) + 1.0));
+            }
+            if (lo) {  // interface at w+0.5
+                vc = std::max(vc, prof((w + 0.5) - i));
+                vh = std::max(vh, prof((double)(w - i)));
+            }
+            zc[(size_t)i] = (C)vc;
+            zh[(size_t)i] = (C)vh;
+        }
+        HIP_TRY(upload(s->d_z[d], zc, s->stream));
+        HIP_TRY(upload(s->d_zh[d], zh, s->stream));
+        std::array<int, 6> full = {1, r.nx - 1, 1, r.ny - 1, 1, r.nz - 1};
+        for (int side = 0; side < 2; ++side) {
+            if (!(side ? hi : lo)) continue;
+            std::array<int, 6> slab = full;
+            if (side) slab[2 * d] = n - 1 - w;
+            else slab[2 * d + 1] = 1 + w;
+            std::vector<std::array<int, 6>> pieces = {slab};
+            for (const auto& b : boxes) {
+                std::vector<std::array<int, 6>> next;
+                for (const auto& p : pieces) subtract_box(p, b, next);
+                pieces.swap(next);
+            }
+            for (const auto& p : pieces)
+                if (p[1] > p[0] && p[3] > p[2] && p[5] > p[4]) boxes.push_back(p);
+        }
+    }
+    std::vector<int32_t> bx;
+    s->box_cells_max = 0;
+    for (const auto& b : boxes) {
+        bx.push_back(0);
+        for (int k = 0; k < 6; ++k) bx.push_back(b[(size_t)k]);
+        s->box_cells_max = std::max<int64_t>(s->box_cells_max, (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]));
+    }
+    s->nbox = (int32_t)boxes.size();
+    HIP_TRY(upload(s->d_boxes, bx, s->stream));
+    std::vector<int32_t> aoff(3 * s->rooms.size(), 0);
+    HIP_TRY(upload(s->d_axis_off, aoff, s->stream));
+    for (int d = 0; d < 3; ++d) {
+        HIP_TRY(s->d_phi[d].alloc(cbytes));
+        HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)cbytes, s->stream));
+    }
+    HIP_TRY(s->d_psi.alloc(cbytes));
+    HIP_TRY(launch_fill(s->d_psi.p, (int64_t)cbytes, s->stream));
+    return AFX_OK;
+}
+
+int commit(afx_sim* s)
+{
+    if (s->committed) {
+        if (s->dirty_special) return build_special(s);
+        return AFX_OK;
+    }
+    if (s->rooms.empty() || s->rooms[0].nx == 0) {
+        // grid only from minsize (no model): simulation.hpp:279-284
+        if (!s->min_size_set) return fail(AFX_E_STATE, "no grid: load a model, set voxels or call afx_set_min_size");
+    }
+    Room& r0 = s->rooms[0];
+    if (s->min_size_set) {
+        r0.nx = s->min_dims[0];
+        r0.ny = s->min_dims[1];
+        r0.nz = s->min_dims[2];
+        if (!r0.vox.empty() && (r0.vx > r0.nx || r0.vy > r0.ny || r0.vz > r0.nz))
+            return fail(AFX_E_RANGE, "minsize grid %dx%dx%d is smaller than the model volume %dx%dx%d", r0.nx, r0.ny,
+                        r0.nz, r0.vx, r0.vy, r0.vz);
+    }
+    for (auto& r : s->rooms)
+        if (r.nx < 3 || r.ny < 3 || r.nz < 3) return fail(AFX_E_ARG, "grid %dx%dx%d too small", r.nx, r.ny, r.nz);
+    if (s->has_model && s->ncoef <= 0) return fail(AFX_E_STATE, "voxels set but no materials");
+    if (s->ncoef > kMaxCoef) return fail(AFX_E_ARG, "ncoef %d above the supported %d", s->ncoef, kMaxCoef);
+    if ((int)s->rooms.size() > kMaxRooms) return fail(AFX_E_ARG, "more than %d rooms", kMaxRooms);
+    int64_t base = 0;
+    for (auto& r : s->rooms) {
+        r.base = base;
+        base += (int64_t)r.nx * r.ny * r.nz;
+    }
+    s->total_cells = base;
+    if (base >= (int64_t)INT32_MAX - 1) return fail(AFX_E_RANGE, "%lld cells exceed the 2^31 index range", (long long)base);
+    if (s->pml_mode == 1 && s->rooms.size() > 1) return fail(AFX_E_ARG, "compat PML supports a single room");
+    if (s->pml_mode == 2) {
+        const Room& r = s->rooms[0];
+        if (2 * s->pml_width + 2 >= std::min(r.nx, std::min(r.ny, r.nz)))
+            return fail(AFX_E_ARG, "PML width %d too large for the %dx%dx%d grid", s->pml_width, r.nx, r.ny, r.nz);
+    }
+    // boundary entries, sorted by cell
+    std::fill(std::begin(s->group_count), std::end(s->group_count), 0);
+    s->entries.clear();
+    if (s->has_model)
+        for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+            int rc = classify_room(s, (int)ri, s->entries);
+            if (rc) return rc;
+        }
+    std::sort(s->entries.begin(), s->entries.end(), [](const Entry& a, const Entry& b) { return a.cell < b.cell; });
+    s->nb = (int32_t)s->entries.size();
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<RoomDev> rd;
+    for (auto& r : s->rooms) {
+        RoomDev d;
+        d.base = r.base;
+        d.nx = r.nx;
+        d.ny = r.ny;
+        d.nz = r.nz;
+        d.pad = 0;
+        rd.push_back(d);
+    }
+    HIP_TRY(upload(s->d_rooms, rd, s->stream));
+    std::vector<int32_t> ecell((size_t)s->nb);
+    std::vector<uint32_t> emat((size_t)s->nb);
+    for (int32_t e = 0; e < s->nb; ++e) {
+        ecell[(size_t)e] = (int32_t)s->entries[(size_t)e].cell;
+        emat[(size_t)e] = s->entries[(size_t)e].mat;
+    }
+    HIP_TRY(upload(s->d_e_cell, ecell, s->stream));
+    HIP_TRY(upload(s->d_e_mat, emat, s->stream));
+    const size_t cbytes = s->prec == Prec::F64 ? 8 : 4;
+    const int N = std::max(s->ncoef, 1);
+    const size_t rbytes = cbytes * (size_t)3 * N * (size_t)std::max(s->nb, 1);
+    HIP_TRY(s->d_ring_in.alloc(rbytes));
+    HIP_TRY(s->d_ring_out.alloc(rbytes));
+    HIP_TRY(launch_fill(s->d_ring_in.p, (int64_t)rbytes, s->stream));
+    HIP_TRY(launch_fill(s->d_ring_out.p, (int64_t)rbytes, s->stream));
+    if (s->prec == Prec::F64) {
+        std::vector<double> a(s->A.begin(), s->A.end()), b(s->B.begin(), s->B.end());
+        if (a.empty()) { a.push_back(1.0); b.push_back(1.0); }
+        HIP_TRY(upload(s->d_A, a, s->stream));
+        HIP_TRY(upload(s->d_B, b, s->stream));
+    } else {
+        std::vector<float> a = s->A, b = s->B;
+        if (a.empty()) { a.push_back(1.0f); b.push_back(1.0f); }
+        HIP_TRY(upload(s->d_A, a, s->stream));
+        HIP_TRY(upload(s->d_B, b, s->stream));
+    }
+    // fields
+    const size_t fbytes = prec_bytes(s->prec) * (size_t)s->total_cells;
+    for (auto& p : s->P) {
+        HIP_TRY(p.alloc(fbytes));
+        HIP_TRY(launch_fill(p.p, (int64_t)((fbytes + 3) / 4 * 4), s->stream));
+    }
+    s->rot[0] = 0; s->rot[1] = 1; s->rot[2] = 2;
+    HIP_TRY(s->d_t.alloc(sizeof(int64_t)));
+    HIP_TRY(launch_fill(s->d_t.p, 8, s->stream));
+    s->t = 0;
+    int rc = build_tile_tables(s);
+    if (rc) return rc;
+    rc = build_special(s);
+    if (rc) return rc;
+    if (s->pml_mode) {
+        rc = s->prec == Prec::F64 ? setup_pml_t<double>(s) : setup_pml_t<float>(s);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->committed = true;
+    return AFX_OK;
+}
+
+int ensure_traces(afx_sim* s, int64_t t_end)
+{
+    if (s->mics.empty()) return AFX_OK;
+    if (t_end <= s->trace_cap && s->d_traces.p) return AFX_OK;
+    int64_t cap = std::max<int64_t>(t_end, s->trace_cap * 2);
+    cap = (cap + 255) / 256 * 256;
+    const size_t nm = s->mics.size();
+    void* np = nullptr;
+    HIP_TRY(hipMalloc(&np, sizeof(float) * nm * (size_t)cap));
+    HIP_TRY(launch_fill(np, (int64_t)(sizeof(float) * nm * (size_t)cap), s->stream));
+    if (s->d_traces.p && s->trace_cap > 0)
+        HIP_TRY(hipMemcpy2DAsync(np, sizeof(float) * (size_t)cap, s->d_traces.p, sizeof(float) * (size_t)s->trace_cap,
+                                 sizeof(float) * (size_t)std::min<int64_t>(s->trace_cap, s->t), nm,
+                                 hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->d_traces.release();
+    s->d_traces.p = np;
+    s->d_traces.bytes = sizeof(float) * nm * (size_t)cap;
+    s->trace_cap = cap;
+    s->drop_graphs();
+    return AFX_OK;
+}
+
+StepArgs make_args(afx_sim* s)
+{
+    StepArgs a{};
+    a.prec = s->prec;
+    a.tile = s->tile;
+    a.out = s->P[s->rot[0]].p;
+    a.cur = s->P[s->rot[1]].p;
+    a.prev = s->P[s->rot[2]].p;
+    a.tiles = s->d_tiles.as<int4>();
+    a.ntiles = s->ntiles;
+    a.rooms = s->d_rooms.as<RoomDev>();
+    // lambda^2 in storage-arithmetic precision exactly as simulation.hpp:166
+    if (s->prec == Prec::F64) {
+        const double c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
+        const double lam = dt * c / dx;
+        a.l2 = lam * lam;
+        a.c2 = 2.0 * (1.0 - 3.0 * a.l2);
+        a.lam_b = s->compat ? a.l2 : lam;
+        a.l2b = a.lam_b * a.lam_b;
+        a.c2b = 2.0 * (1.0 - 3.0 * a.l2b);
+    } else {
+        const float c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
+        const float lam = dt * c / dx;
+        const float l2 = (dt * c / dx) * (dt * c / dx);
+        const float lb = s->compat ? l2 : lam;
+        const float l2b = lb * lb;
+        a.l2 = l2;
+        a.c2 = 2.0f * (1.0f - 3.0f * l2);
+        a.lam_b = lb;
+        a.l2b = l2b;
+        a.c2b = 2.0f * (1.0f - 3.0f * l2b);
+    }
+    a.compat = s->compat ? 1 : 0;
+    a.nb = s->nb;
+    a.e_cell = s->d_e_cell.as<int32_t>();
+    a.e_info = s->d_e_info.as<uint32_t>();
+    a.e_mat = s->d_e_mat.as<uint32_t>();
+    a.ring_in = s->d_ring_in.p;
+    a.ring_out = s->d_ring_out.p;
+    a.coef_a = s->d_A.p;
+    a.coef_b = s->d_B.p;
+    a.ncoef = s->has_model ? s->ncoef : 0;
+    a.nsp = s->nsp;
+    a.sp_cell = s->d_sp_cell.as<int32_t>();
+    a.sp_room = s->d_sp_room.as<int32_t>();
+    a.sp_bidx = s->d_sp_bidx.as<int32_t>();
+    a.sp_src_off = s->d_sp_src_off.as<int32_t>();
+    a.sp_src = s->d_sp_src.as<int32_t>();
+    a.sp_mic_off = s->d_sp_mic_off.as<int32_t>();
+    a.sp_mic = s->d_sp_mic.as<int32_t>();
+    a.src = s->d_src.as<float>();
+    a.src_cap = s->src_cap;
+    a.src_len = s->d_src_len.as<int64_t>();
+    a.traces = s->d_traces.as<float>();
+    a.trace_cap = s->trace_cap;
+    a.scale = s->scale;
+    a.d_t = s->d_t.as<int64_t>();
+    a.t_rel = 0;
+    return a;
+}
+
+PmlArgs make_pml(afx_sim* s, const StepArgs& st)
+{
+    PmlArgs p{};
+    p.prec = s->prec;
+    p.out = st.out;
+    p.cur = st.cur;
+    p.prev = st.prev;
+    for (int d = 0; d < 3; ++d) {
+        p.phi[d] = s->d_phi[d].p;
+        p.phi_old[d] = s->d_phi_old[d].p;
+        p.z[d] = s->d_z[d].p;
+        p.zh[d] = s->d_zh[d].p;
+    }
+    p.psi = s->d_psi.p;
+    p.psi_old = s->d_psi_old.p;
+    if (s->pml_mode == 1 && s->pml_rot) {
+        for (int d = 0; d < 3; ++d) std::swap(p.phi[d], p.phi_old[d]);
+        std::swap(p.psi, p.psi_old);
+    }
+    p.boxes = s->d_boxes.as<int32_t>();
+    p.nbox = s->nbox;
+    p.box_cells_max = s->box_cells_max;
+    p.rooms = s->d_rooms.as<RoomDev>();
+    p.nrooms = (int32_t)s->rooms.size();
+    p.axis_off = s->d_axis_off.as<int32_t>();
+    p.l2 = st.l2;
+    p.c = s->cfg.c;
+    p.dx = s->cfg.dx;
+    p.dt = s->cfg.dt;
+    p.width = s->pml_width;
+    return p;
+}
+
+bool use_fused(const afx_sim* s) { return s->pml_mode == 0 && !(s->cfg.flags & AFX_SPLIT) && s->fused_ok; }
+
+// Enqueue `k` steps starting at the device counter, then advance it by k.
+// Buffer roles rotate like simulation.hpp:360.  local0 is the step index
+// inside the current afx_run call (compat PML state reset, :411-430).
+struct Prof {
+    std::vector<hipEvent_t> ev;  // pairs
+    std::vector<int> cls;
+    size_t used = 0;
+    hipError_t mark(int klass, hipStream_t st, bool begin)
+    {
+        if (begin) {
+            if (used + 2 > ev.size()) {
+                for (int q = 0; q < 2; ++q) {
+                    hipEvent_t e;
+                    hipError_t r = hipEventCreate(&e);
+                    if (r != hipSuccess) return r;
+                    ev.push_back(e);
+                }
+                cls.push_back(klass);
+            } else {
+                cls[used / 2] = klass;
+            }
+            return hipEventRecord(ev[used], st);
+        }
+        hipError_t r = hipEventRecord(ev[used + 1], st);
+        used += 2;
+        return r;
+    }
+    ~Prof()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+#define PROF_BEGIN(k) \
+    if (prof) HIP_TRY(prof->mark((k), s->stream, true))
+#define PROF_END(k) \
+    if (prof) HIP_TRY(prof->mark((k), s->stream, false))
+
+int enqueue_steps(afx_sim* s, int k, int64_t local0, Prof* prof = nullptr)
+{
+    for (int j = 0; j < k; ++j) {
+        const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2];
+        s->rot[0] = r2;
+        s->rot[1] = r0;
+        s->rot[2] = r1;
+        StepArgs a = make_args(s);
+        a.t_rel = j;
+        if (use_fused(s)) {
+            PROF_BEGIN(0);
+            HIP_TRY(launch_fused(a, s->d_tile_eoff.as<int32_t>(), s->d_tile_eperm.as<int32_t>(),
+                                 s->d_tile_spoff.as<int32_t>(), s->stream));
+            PROF_END(0);
+            continue;
+        }
+        PROF_BEGIN(0);
+        HIP_TRY(launch_interior(a, s->stream));
+        PROF_END(0);
+        if (s->pml_mode == 0) {
+            PROF_BEGIN(1);
+            HIP_TRY(launch_boundary(a, 1, 1, s->stream));
+            PROF_END(1);
+        } else if (s->pml_mode == 2) {
+            PmlArgs p = make_pml(s, a);
+            PROF_BEGIN(2);
+            HIP_TRY(launch_pml_phys_p(p, s->stream));
+            PROF_END(2);
+            PROF_BEGIN(1);
+            HIP_TRY(launch_boundary(a, 0, 0, s->stream));
+            StepArgs sp = a;
+            sp.nb = 0;
+            HIP_TRY(launch_boundary(sp, 0, 2, s->stream));
+            PROF_END(1);
+            PROF_BEGIN(3);
+            HIP_TRY(launch_pml_phys_phi(p, s->stream));
+            PROF_END(3);
+        } else {
+            PROF_BEGIN(1);
+            HIP_TRY(launch_boundary(a, 0, 0, s->stream));
+            PROF_END(1);
+            s->pml_rot ^= 1;
+            PmlArgs p = make_pml(s, a);
+            PROF_BEGIN(2);
+            HIP_TRY(launch_pml_compat(p, s->stream));
+            PROF_END(2);
+            StepArgs sp = a;
+            sp.nb = 0;
+            PROF_BEGIN(1);
+            HIP_TRY(launch_boundary(sp, 0, 2, s->stream));
+            PROF_END(1);
+            if ((local0 + j) % 8000 == 0) {
+                const size_t cb = (s->prec == Prec::F64 ? 8 : 4) * (size_t)s->total_cells;
+                for (int d = 0; d < 3; ++d) {
+                    HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)cb, s->stream));
+                    HIP_TRY(launch_fill(s->d_phi_old[d].p, (int64_t)cb, s->stream));
+                }
+                HIP_TRY(launch_fill(s->d_psi.p, (int64_t)cb, s->stream));
+                HIP_TRY(launch_fill(s->d_psi_old.p, (int64_t)cb, s->stream));
+            }
+        }
+    }
+    HIP_TRY(launch_advance_t(s->d_t.as<int64_t>(), k, s->stream));
+    s->t += k;
+    return AFX_OK;
+}
+
+int run_steps(afx_sim* s, int64_t steps)
+{
+    int rc = ensure_traces(s, s->t + steps);
+    if (rc) return rc;
+    const bool graphs = !(s->cfg.flags & AFX_NO_GRAPH) && s->pml_mode != 1;
+    const int K = s->graph_steps;  // multiple of 3: buffer roles repeat
+    int64_t done = 0;
+    while (done < steps) {
+        const int64_t left = steps - done;
+        if (graphs && left >= K) {
+            const int ri = s->rot[0];  // rotation phase at chunk start
+            if (!s->graph[ri]) {
+                int saved_rot[3] = {s->rot[0], s->rot[1], s->rot[2]};
+                const int64_t saved_t = s->t;
+                hipGraph_t g = nullptr;
+                HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                int erc = enqueue_steps(s, K, done);
+                hipError_t ce = hipStreamEndCapture(s->stream, &g);
+                if (erc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return erc;
+                }
+                HIP_TRY(ce);
+                hipError_t ie = hipGraphInstantiate(&s->graph[ri], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_TRY(ie);
+                // capture did not execute anything: restore the host mirrors
+                s->rot[0] = saved_rot[0];
+                s->rot[1] = saved_rot[1];
+                s->rot[2] = saved_rot[2];
+                s->t = saved_t;
+            }
+            HIP_TRY(hipGraphLaunch(s->graph[ri], s->stream));
+            // advance host mirrors as the graph did
+            for (int j = 0; j < K; ++j) {
+                const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2];
+                s->rot[0] = r2;
+                s->rot[1] = r0;
+                s->rot[2] = r1;
+            }
+            s->t += K;
+            done += K;
+        } else {
+            const int k = (int)std::min<int64_t>(left, graphs ? (int64_t)1 : (int64_t)64);
+            rc = enqueue_steps(s, k, done);
+            if (rc) return rc;
+            done += k;
+        }
+    }
+    return AFX_OK;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+
+extern "C" {
+
+int afx_config_default(afx_config* cfg)
+{
+    if (!cfg) return fail(AFX_E_ARG, "null config");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->c = 340.0f;
+    cfg->dx = 0.02f;
+    cfg->dt = 3.125e-05f;
+    cfg->res = 0.02f;
+    cfg->precision = AFX_FP32;
+    cfg->flags = AFX_COMPAT;
+    cfg->device = 0;
+    cfg->pml_order = 3.0f;
+    cfg->pml_r0 = 1e-3f;
+    cfg->scale = 0.0f;
+    return AFX_OK;
+}
+
+int afx_device_count(int32_t* n)
+{
+    if (!n) return fail(AFX_E_ARG, "null pointer");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *n = 0;
+        return fail(AFX_E_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *n = c;
+    return AFX_OK;
+}
+
+int afx_device_name(int32_t i, char* buf, size_t len)
+{
+    if (!buf || len == 0) return fail(AFX_E_ARG, "null buffer");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, i));
+    std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return AFX_OK;
+}
+
+const char* afx_last_error(void) { return g_err.c_str(); }
+
+int afx_create(const afx_config* cfg, afx_sim** out)
+{
+    if (!cfg || !out) return fail(AFX_E_ARG, "null argument");
+    *out = nullptr;
+    if (!(cfg->c > 0) || !(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->res > 0))
+        return fail(AFX_E_ARG, "c, dx, dt and res must be positive");
+    if (cfg->precision < AFX_FP32 || cfg->precision > AFX_BF16) return fail(AFX_E_ARG, "unknown precision %d", cfg->precision);
+    const double lam = (double)cfg->dt * cfg->c / cfg->dx;
+    if (lam > 1.0 / std::sqrt(3.0) + 1e-7)
+        return fail(AFX_E_ARG, "Courant number %.6f above the 3-D stability limit 1/sqrt(3)", lam);
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(AFX_E_HIP, "no HIP device available");
+    if (cfg->device < 0 || cfg->device >= n) return fail(AFX_E_ARG, "device %d outside 0..%d", cfg->device, n - 1);
+    std::unique_ptr<afx_sim> s(new (std::nothrow) afx_sim());
+    if (!s) return fail(AFX_E_NOMEM, "out of host memory");
+    s->cfg = *cfg;
+    s->prec = (Prec)cfg->precision;
+    s->compat = (cfg->flags & AFX_COMPAT) != 0;
+    HIP_TRY(hipSetDevice(cfg->device));
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->rooms.resize(1);
+    *out = s.release();
+    return AFX_OK;
+}
+
+void afx_destroy(afx_sim* sim)
+{
+    if (!sim) return;
+    (void)hipSetDevice(sim->cfg.device);
+    if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+    delete sim;
+}
+
+int afx_set_voxels(afx_sim* s, int32_t nx, int32_t ny, int32_t nz, const int32_t* vox)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (s->committed) return fail(AFX_E_STATE, "geometry is fixed after the first run");
+    if (nx < 3 || ny < 3 || nz < 3) return fail(AFX_E_ARG, "grid %dx%dx%d too small", nx, ny, nz);
+    Room& r = s->rooms[0];
+    r.nx = r.vx = nx;
+    r.ny = r.vy = ny;
+    r.nz = r.vz = nz;
+    r.vox.clear();
+    if (vox) {
+        r.vox.assign(vox, vox + (size_t)nx * ny * nz);
+        s->has_model = true;
+    }
+    return AFX_OK;
+}
+
+int afx_set_materials(afx_sim* s, const float* a, const float* b, int32_t ncoef, int32_t nmat)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (s->committed) return fail(AFX_E_STATE, "materials are fixed after the first run");
+    if (!a || !b || ncoef < 1 || nmat < 1) return fail(AFX_E_ARG, "materials need a, b, ncoef >= 1, nmat >= 1");
+    if (ncoef > kMaxCoef) return fail(AFX_E_ARG, "ncoef %d above the supported %d", ncoef, kMaxCoef);
+    if (s->rooms.size() > 1 && ncoef != s->ncoef) return fail(AFX_E_ARG, "all rooms share ncoef");
+    for (int32_t i = 0; i < ncoef * nmat; i += ncoef)
+        if (a[i] == 0.0f || b[i] == 0.0f) return fail(AFX_E_ARG, "material %d: a[0] and b[0] must be non-zero", i / ncoef);
+    // room 0's materials occupy ids [0, nmat); other rooms follow
+    std::vector<float> A(a, a + (size_t)ncoef * nmat), B(b, b + (size_t)ncoef * nmat);
+    const int32_t old = s->rooms[0].nmat;
+    const int32_t shift = nmat - old;
+    if (s->rooms.size() > 1) {
+        A.insert(A.end(), s->A.begin() + (size_t)old * ncoef, s->A.end());
+        B.insert(B.end(), s->B.begin() + (size_t)old * ncoef, s->B.end());
+        for (size_t i = 1; i < s->rooms.size(); ++i) s->rooms[i].mat_base += shift;
+    }
+    s->A.swap(A);
+    s->B.swap(B);
+    s->ncoef = ncoef;
+    s->rooms[0].nmat = nmat;
+    if ((int)(s->A.size() / (size_t)ncoef) > kMaxMaterials) return fail(AFX_E_ARG, "more than %d materials", kMaxMaterials);
+    return AFX_OK;
+}
+
+int afx_load_kerkythea(afx_sim* s, const char* path)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!path) return fail(AFX_E_ARG, "null path");
+    if (s->committed) return fail(AFX_E_STATE, "geometry is fixed after the first run");
+    int32_t dims[3];
+    std::vector<int32_t> vox;
+    std::vector<float> a, b;
+    int32_t ncoef = 0, nmat = 0;
+    std::string err;
+    int rc = afx::load_kerkythea(path, s->cfg.res, dims, vox, a, b, &ncoef, &nmat, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    Room& r = s->rooms[0];
+    // simulation.hpp:180-182: the grid grows to the model's volume
+    r.vx = dims[0];
+    r.vy = dims[1];
+    r.vz = dims[2];
+    r.nx = std::max(r.nx, dims[0]);
+    r.ny = std::max(r.ny, dims[1]);
+    r.nz = std::max(r.nz, dims[2]);
+    r.vox.swap(vox);
+    s->has_model = true;
+    if (s->min_size_set) {
+        // a later Load keeps the larger of the two (simulation.hpp:180-182)
+        s->min_dims[0] = std::max(s->min_dims[0], dims[0]);
+        s->min_dims[1] = std::max(s->min_dims[1], dims[1]);
+        s->min_dims[2] = std::max(s->min_dims[2], dims[2]);
+    }
+    return afx_set_materials(s, a.data(), b.data(), ncoef, nmat);
+}
+
+int afx_set_min_size(afx_sim* s, float x, float y, float z)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (s->committed) return fail(AFX_E_STATE, "geometry is fixed after the first run");
+    if (!(x >= 0) || !(y >= 0) || !(z >= 0)) return fail(AFX_E_ARG, "negative size");
+    const float res = s->cfg.res;
+    s->min_dims[0] = (int32_t)round_to((uint32_t)(size_t)(x / res) + 1, 32);
+    s->min_dims[1] = (int32_t)round_to((uint32_t)(size_t)(y / res) + 1, 32);
+    s->min_dims[2] = (int32_t)round_to((uint32_t)(size_t)(z / res) + 1, 32);
+    s->min_size_set = true;
+    return AFX_OK;
+}
+
+int afx_set_pml(afx_sim* s, int32_t width, uint32_t face_mask, const float* profile, int32_t len)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (s->committed) return fail(AFX_E_STATE, "the PML is fixed after the first run");
+    if (width < 0) return fail(AFX_E_ARG, "negative PML width");
+    s->pml_width = width;
+    s->pml_faces = face_mask & 63u;
+    s->pml_profile.clear();
+    if (profile && len > 0) s->pml_profile.assign(profile, profile + len);
+    if (width == 0) {
+        s->pml_mode = 0;  // simulation.hpp:373: width 0 disables the PML
+        return AFX_OK;
+    }
+    if (s->compat) {
+        s->pml_mode = 1;
+    } else {
+        if (!s->pml_faces) return fail(AFX_E_ARG, "physical PML needs at least one face");
+        if (profile && len != width) return fail(AFX_E_ARG, "profile length %d differs from width %d", len, width);
+        s->pml_mode = 2;
+    }
+    return AFX_OK;
+}
+
+static int add_source_impl(afx_sim* s, int32_t room, const int32_t c[3], const float* samples, int64_t n)
+{
+    if (room < 0 || room >= (int32_t)s->rooms.size()) return fail(AFX_E_ARG, "room %d does not exist", room);
+    if (n < 0 || (n > 0 && !samples)) return fail(AFX_E_ARG, "bad source samples");
+    Source q;
+    q.room = room;
+    q.cell[0] = c[0];
+    q.cell[1] = c[1];
+    q.cell[2] = c[2];
+    q.samples.assign(samples, samples + n);
+    s->srcs.push_back(std::move(q));
+    if (s->committed) s->dirty_special = true;
+    return AFX_OK;
+}
+
+static int add_mic_impl(afx_sim* s, int32_t room, const int32_t c[3])
+{
+    if (room < 0 || room >= (int32_t)s->rooms.size()) return fail(AFX_E_ARG, "room %d does not exist", room);
+    Mic m;
+    m.room = room;
+    m.cell[0] = c[0];
+    m.cell[1] = c[1];
+    m.cell[2] = c[2];
+    if (s->committed) {
+        // grow the trace buffer by one row, keeping earlier traces
+        const size_t nm = s->mics.size();
+        if (s->d_traces.p && s->trace_cap > 0) {
+            void* np = nullptr;
+            HIP_TRY(hipMalloc(&np, sizeof(float) * (nm + 1) * (size_t)s->trace_cap));
+            HIP_TRY(launch_fill(np, (int64_t)(sizeof(float) * (nm + 1) * (size_t)s->trace_cap), s->stream));
+            HIP_TRY(hipMemcpyAsync(np, s->d_traces.p, sizeof(float) * nm * (size_t)s->trace_cap, hipMemcpyDeviceToDevice,
+                                   s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            s->d_traces.release();
+            s->d_traces.p = np;
+            s->d_traces.bytes = sizeof(float) * (nm + 1) * (size_t)s->trace_cap;
+        }
+        s->dirty_special = true;
+    }
+    s->mics.push_back(m);
+    return AFX_OK;
+}
+
+int afx_add_source(afx_sim* s, const float pos[3], const float* samples, int64_t n)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!pos) return fail(AFX_E_ARG, "null position");
+    const float res = s->cfg.res;
+    // simulation.hpp:322: (cl_int)(pos / res_) in single precision
+    const int32_t c[3] = {(int32_t)(pos[0] / res), (int32_t)(pos[1] / res), (int32_t)(pos[2] / res)};
+    return add_source_impl(s, 0, c, samples, n);
+}
+
+int afx_add_receiver(afx_sim* s, const float pos[3])
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!pos) return fail(AFX_E_ARG, "null position");
+    const float res = s->cfg.res;
+    const int32_t c[3] = {(int32_t)(pos[0] / res), (int32_t)(pos[1] / res), (int32_t)(pos[2] / res)};
+    return add_mic_impl(s, 0, c);
+}
+
+int afx_add_source_cell(afx_sim* s, int32_t room, const int32_t cell[3], const float* samples, int64_t n)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!cell) return fail(AFX_E_ARG, "null cell");
+    return add_source_impl(s, room, cell, samples, n);
+}
+
+int afx_add_receiver_cell(afx_sim* s, int32_t room, const int32_t cell[3])
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!cell) return fail(AFX_E_ARG, "null cell");
+    return add_mic_impl(s, room, cell);
+}
+
+int afx_add_room(afx_sim* s, int32_t nx, int32_t ny, int32_t nz, const int32_t* vox, const float* a, const float* b,
+                 int32_t ncoef, int32_t nmat, int32_t* room_id)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (s->committed) return fail(AFX_E_STATE, "rooms are fixed after the first run");
+    if (nx < 3 || ny < 3 || nz < 3) return fail(AFX_E_ARG, "grid %dx%dx%d too small", nx, ny, nz);
+    if (s->rooms[0].nx == 0) return fail(AFX_E_STATE, "set room 0 first (afx_set_voxels)");
+    if (vox && (!a || !b || ncoef < 1 || nmat < 1)) return fail(AFX_E_ARG, "a room with voxels needs materials");
+    if (vox && s->ncoef && ncoef != s->ncoef) return fail(AFX_E_ARG, "all rooms share ncoef (%d)", s->ncoef);
+    if ((int)s->rooms.size() >= kMaxRooms) return fail(AFX_E_ARG, "more than %d rooms", kMaxRooms);
+    Room r;
+    r.nx = r.vx = nx;
+    r.ny = r.vy = ny;
+    r.nz = r.vz = nz;
+    if (vox) {
+        r.vox.assign(vox, vox + (size_t)nx * ny * nz);
+        for (int32_t i = 0; i < ncoef * nmat; i += ncoef)
+            if (a[i] == 0.0f || b[i] == 0.0f) return fail(AFX_E_ARG, "material %d: a[0] and b[0] must be non-zero", i / ncoef);
+        r.mat_base = s->ncoef ? (int32_t)(s->A.size() / (size_t)s->ncoef) : 0;
+        r.nmat = nmat;
+        s->ncoef = ncoef;
+        s->A.insert(s->A.end(), a, a + (size_t)ncoef * nmat);
+        s->B.insert(s->B.end(), b, b + (size_t)ncoef * nmat);
+        s->has_model = true;
+        if ((int)(s->A.size() / (size_t)ncoef) > kMaxMaterials) return fail(AFX_E_ARG, "more than %d materials", kMaxMaterials);
+    }
+    s->rooms.push_back(std::move(r));
+    if (room_id) *room_id = (int32_t)s->rooms.size() - 1;
+    return AFX_OK;
+}
+
+int afx_run(afx_sim* s, int64_t steps)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (steps < 0) return fail(AFX_E_ARG, "negative step count");
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    if (int rc = commit(s)) return rc;
+    if (steps == 0) return AFX_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = run_steps(s, steps);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    s->last_run_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return AFX_OK;
+}
+
+int afx_get_receivers(afx_sim* s, float* out, int64_t cap, int64_t* n_written)
+{
+    if (int rc = check_sim(s)) return rc;
+    const int64_t nm = (int64_t)s->mics.size();
+    const int64_t T = s->t;
+    if (n_written) *n_written = nm * T;
+    if (nm == 0 || T == 0) return AFX_OK;
+    if (!out || cap < nm * T) return fail(AFX_E_ARG, "buffer holds %lld values, %lld needed", (long long)cap, (long long)(nm * T));
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy2D(out, sizeof(float) * (size_t)T, s->d_traces.p, sizeof(float) * (size_t)s->trace_cap,
+                        sizeof(float) * (size_t)T, (size_t)nm, hipMemcpyDeviceToHost));
+    return AFX_OK;
+}
+
+int afx_get_pressure(afx_sim* s, int32_t room, int32_t which, float* out, int64_t cap)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!s->committed) return fail(AFX_E_STATE, "no field before the first run");
+    if (room < 0 || room >= (int32_t)s->rooms.size()) return fail(AFX_E_ARG, "room %d does not exist", room);
+    if (which < 0 || which > 2) return fail(AFX_E_ARG, "which must be 0, 1 or 2");
+    const Room& r = s->rooms[(size_t)room];
+    const int64_t n = (int64_t)r.nx * r.ny * r.nz;
+    if (!out || cap < n) return fail(AFX_E_ARG, "buffer holds %lld values, %lld needed", (long long)cap, (long long)n);
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    DBuf tmp;
+    HIP_TRY(tmp.alloc(sizeof(float) * (size_t)n));
+    const char* src = static_cast<const char*>(s->P[s->rot[which]].p) + prec_bytes(s->prec) * (size_t)r.base;
+    HIP_TRY(launch_to_f32(src, s->prec, tmp.as<float>(), n, s->scale, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, tmp.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return AFX_OK;
+}
+
+int afx_get_stats(afx_sim* s, afx_stats* st)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!st) return fail(AFX_E_ARG, "null stats");
+    std::memset(st, 0, sizeof *st);
+    st->cells = s->total_cells;
+    st->boundary = s->nb;
+    for (int g = 0; g < 26; ++g) {
+        st->group_count[g] = s->group_count[g];
+        if (g <= 5) st->faces += s->group_count[g];
+        else if (g <= 17) st->edges += s->group_count[g];
+        else st->corners += s->group_count[g];
+    }
+    st->steps = s->t;
+    st->rooms = (int32_t)s->rooms.size();
+    st->nx = s->rooms[0].nx;
+    st->ny = s->rooms[0].ny;
+    st->nz = s->rooms[0].nz;
+    st->tile_x = kTiles[s->tile].tx;
+    st->tile_y = kTiles[s->tile].ty;
+    st->tile_z = kTiles[s->tile].tz;
+    st->fused = use_fused(s) ? 1 : 0;
+    st->last_run_ms = s->last_run_ms;
+    return AFX_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== profiler
+extern "C" int afx_profile_run(afx_sim* s, int64_t steps, afx_profile* out)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!out || steps < 1) return fail(AFX_E_ARG, "need an output struct and steps >= 1");
+    std::memset(out, 0, sizeof *out);
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    if (int rc = commit(s)) return rc;
+    if (int rc = ensure_traces(s, s->t + steps)) return rc;
+    Prof prof;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int64_t done = 0;
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    while (done < steps) {
+        const int k = (int)std::min<int64_t>(steps - done, 64);
+        if (int rc = enqueue_steps(s, k, done, &prof)) return rc;
+        done += k;
+        HIP_TRY(hipEventRecord(e1, s->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        for (size_t q = 0; q < prof.used; q += 2) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, prof.ev[q], prof.ev[q + 1]));
+            const int kl = prof.cls[q / 2];
+            out->kernel_ms[kl] += ms;
+            out->launches[kl] += 1;
+        }
+        prof.used = 0;
+    }
+    float tot = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&tot, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int kl = 0; kl < 4; ++kl)
+        if (out->launches[kl]) out->kernel_ms[kl] /= (double)out->launches[kl];
+    out->step_ms = tot / (double)steps;
+    out->steps = steps;
+    return AFX_OK;
+}
+
+// ============================================================== autotuner
+namespace {
+
+struct Snapshot {
+    std::vector<std::unique_ptr<DBuf>> bufs;
+    std::vector<const DBuf*> src;
+    int rot[3];
+    int64_t t;
+    int pml_rot;
+};
+
+int snap_take(afx_sim* s, Snapshot& sn)
+{
+    std::vector<const DBuf*> list = {&s->P[0], &s->P[1], &s->P[2], &s->d_ring_in, &s->d_ring_out, &s->d_traces, &s->d_t,
+                                     &s->d_psi, &s->d_psi_old};
+    for (int d = 0; d < 3; ++d) {
+        list.push_back(&s->d_phi[d]);
+        list.push_back(&s->d_phi_old[d]);
+    }
+    for (const DBuf* b : list) {
+        if (!b->p) continue;
+        auto c = std::make_unique<DBuf>();
+        HIP_TRY(c->alloc(b->bytes));
+        HIP_TRY(hipMemcpyAsync(c->p, b->p, b->bytes, hipMemcpyDeviceToDevice, s->stream));
+        sn.bufs.push_back(std::move(c));
+        sn.src.push_back(b);
+    }
+    std::copy(s->rot, s->rot + 3, sn.rot);
+    sn.t = s->t;
+    sn.pml_rot = s->pml_rot;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return AFX_OK;
+}
+
+int snap_restore(afx_sim* s, const Snapshot& sn)
+{
+    for (size_t i = 0; i < sn.bufs.size(); ++i)
+        HIP_TRY(hipMemcpyAsync(sn.src[i]->p, sn.bufs[i]->p, sn.bufs[i]->bytes, hipMemcpyDeviceToDevice, s->stream));
+    std::copy(sn.rot, sn.rot + 3, s->rot);
+    s->t = sn.t;
+    s->pml_rot = sn.pml_rot;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return AFX_OK;
+}
+
+int set_tile(afx_sim* s, int tile)
+{
+    s->tile = tile;
+    s->drop_graphs();
+    int rc = build_tile_tables(s);
+    if (rc) return rc;
+    return build_special(s);
+}
+
+struct TuneKey {
+    std::vector<int32_t> dims;
+    int prec, compat, pml;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(dims, prec, compat, pml) < std::tie(o.dims, o.prec, o.compat, o.pml);
+    }
+};
+std::map<TuneKey, int> g_tune_cache;
+
+}  // namespace
+
+extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3])
+{
+    if (int rc = check_sim(s)) return rc;
+    if (steps_per_trial < 1) return fail(AFX_E_ARG, "steps_per_trial must be positive");
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    if (int rc = commit(s)) return rc;
+    if (int rc = ensure_traces(s, s->t + 2 * (int64_t)steps_per_trial)) return rc;
+    TuneKey key;
+    for (auto& r : s->rooms) {
+        key.dims.push_back(r.nx);
+        key.dims.push_back(r.ny);
+        key.dims.push_back(r.nz);
+    }
+    key.prec = (int)s->prec;
+    key.compat = s->compat;
+    key.pml = s->pml_mode;
+    auto hit = g_tune_cache.find(key);
+    if (hit != g_tune_cache.end()) {
+        if (hit->second != s->tile)
+            if (int rc = set_tile(s, hit->second)) return rc;
+    } else {
+        Snapshot sn;
+        if (int rc = snap_take(s, sn)) return rc;
+        const int base_tile = s->tile;
+        const size_t fbytes = s->P[0].bytes;
+        std::vector<char> ref(fbytes), got(fbytes);
+        int best = base_tile;
+        float best_ms = 1e30f;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        for (int cand = -1; cand < kNumTiles; ++cand) {
+            const int tl = cand < 0 ? base_tile : cand;
+            if (cand == base_tile) continue;
+            if (int rc = snap_restore(s, sn)) return rc;
+            if (int rc = set_tile(s, tl)) return rc;
+            // untimed pass (captures graphs, warms caches), then the check pass
+            if (int rc = run_steps(s, steps_per_trial)) return rc;
+            if (int rc = snap_restore(s, sn)) return rc;
+            HIP_TRY(hipEventRecord(e0, s->stream));
+            if (int rc = run_steps(s, steps_per_trial)) return rc;
+            HIP_TRY(hipEventRecord(e1, s->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            HIP_TRY(hipMemcpy(cand < 0 ? ref.data() : got.data(), s->P[s->rot[0]].p, fbytes, hipMemcpyDeviceToHost));
+            if (cand >= 0 && std::memcmp(ref.data(), got.data(), fbytes) != 0) continue;  // not bit-exact: rejected
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = tl;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (int rc = snap_restore(s, sn)) return rc;
+        if (int rc = set_tile(s, best)) return rc;
+        g_tune_cache[key] = best;
+    }
+    if (tile) {
+        tile[0] = kTiles[s->tile].tx;
+        tile[1] = kTiles[s->tile].ty;
+        tile[2] = kTiles[s->tile].tz;
+    }
+    return AFX_OK;
+}

@@ -1,0 +1,764 @@
+// gfx950 kernels of the FDTD room-acoustics step.
+//
+// Reference counterparts (paths relative to the reference root):
+//   k_interior   anc-field/src/leapfrog.cl:34-55
+//   bnd_value    anc-field/src/lrs_1.cl:116-179, lrs_2.cl:62-215, lrs_3.cl:62-186
+//   special      anc-field/src/source.cl:32-40, mic.cl:32-41 (order simulation.hpp:384-392)
+//   k_fused      the whole step of simulation.hpp:358-392 in one launch
+//   pml_compat   anc-field/src/pml.cl:154-265
+//   pml_phys     no counterpart (Grote-Sim PML restricted to its slabs)
+//
+// Arithmetic follows the reference's operation order and the library is
+// compiled with -ffp-contract=off and IEEE division, so FP32 results are
+// bit-identical to the reference operations evaluated in IEEE single.
+#include "afx_internal.hpp"
+
+namespace afx {
+
+const TileShape kTiles[kNumTiles] = {
+    {64, 4, 16}, {64, 4, 32}, {64, 8, 16}, {64, 2, 32}, {128, 2, 16}, {64, 4, 8}, {256, 1, 16},
+};
+
+// ---------------------------------------------------------------- storage I/O
+template <typename S> struct Compute { using T = float; };
+template <> struct Compute<double> { using T = double; };
+
+template <typename S, typename C>
+__device__ __forceinline__ C ld(const S* __restrict__ p, int64_t i) { return static_cast<C>(p[i]); }
+template <typename S, typename C>
+__device__ __forceinline__ void st(S* __restrict__ p, int64_t i, C v) { p[i] = static_cast<S>(v); }
+// value after a round trip through storage precision (identity for FP32/FP64)
+template <typename S, typename C>
+__device__ __forceinline__ C rt(C v) { return static_cast<C>(static_cast<S>(v)); }
+
+// ---------------------------------------------------------------- interior
+// One thread owns an (x, y) column of a TX x TY x TZ tile and marches z,
+// carrying p^n at z-1, z, z+1 in registers; x and y neighbours come from the
+// same and adjacent 64-lane rows (L1 / L2 hits).  The sum order is the
+// reference's: x+1, x-1, y+1, y-1, z+1, z-1.
+template <typename S, typename C, int TZ>
+__device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __restrict__ cur,
+                                                const S* __restrict__ prev, const RoomDev& r, int x, int y,
+                                                int z0, C l2, C c2)
+{
+    if (x < 1 || x >= r.nx - 1 || y < 1 || y >= r.ny - 1) return;
+    const int zs = z0 < 1 ? 1 : z0;
+    const int ze = (z0 + TZ) < (r.nz - 1) ? (z0 + TZ) : (r.nz - 1);
+    if (zs >= ze) return;
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + x;
+    C pd = ld<S, C>(cur, i - sz);
+    C pc = ld<S, C>(cur, i);
+#pragma unroll 4
+    for (int z = zs; z < ze; ++z, i += sz) {
+        const C pu = ld<S, C>(cur, i + sz);
+        C s = ld<S, C>(cur, i + 1) + ld<S, C>(cur, i - 1);
+        s = s + ld<S, C>(cur, i + sy);
+        s = s + ld<S, C>(cur, i - sy);
+        s = s + pu;
+        s = s + pd;
+        const C v = l2 * s + c2 * pc;
+        st<S, C>(out, i, v - ld<S, C>(prev, i));
+        pd = pc;
+        pc = pu;
+    }
+}
+
+template <typename S, int TX, int TY, int TZ>
+__global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const S* __restrict__ cur,
+                                                      const S* __restrict__ prev, const int4* __restrict__ tiles,
+                                                      const RoomDev* __restrict__ rooms,
+                                                      typename Compute<S>::T l2, typename Compute<S>::T c2)
+{
+    using C = typename Compute<S>::T;
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const RoomDev r = rooms[t.x];
+    const int x = t.y + (int)(threadIdx.x % TX);
+    const int y = t.z + (int)(threadIdx.x / TX);
+    interior_column<S, C, TZ>(out, cur, prev, r, x, y, t.w, l2, c2);
+}
+
+// ---------------------------------------------------------------- boundary
+template <typename C> struct BndScalars {
+    C lam_b, l2b, c2b, l2, c2;
+};
+
+// Leapfrog value of one cell, same operation order as interior_column.
+template <typename S, typename C>
+__device__ __forceinline__ C leapfrog_value(const S* __restrict__ cur, const S* __restrict__ prev, int64_t i,
+                                            int64_t sy, int64_t sz, C l2, C c2)
+{
+    C s = ld<S, C>(cur, i + 1) + ld<S, C>(cur, i - 1);
+    s = s + ld<S, C>(cur, i + sy);
+    s = s + ld<S, C>(cur, i - sy);
+    s = s + ld<S, C>(cur, i + sz);
+    s = s + ld<S, C>(cur, i - sz);
+    const C v = l2 * s + c2 * ld<S, C>(cur, i);
+    return v - ld<S, C>(prev, i);
+}
+
+// New pressure of boundary entry e (rounded to storage precision) with its
+// IIR rings advanced.  Mirrored neighbour sum: axes x, y, z in order; a
+// solid face replaces its pair by 2*opposite (lrs_1.cl:135-175).
+template <typename S, typename C>
+__device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* __restrict__ cur,
+                       const S* __restrict__ prev, C* __restrict__ rin, C* __restrict__ rout, int64_t rs,
+                       const C* __restrict__ A, const C* __restrict__ B, int N, unsigned ixb, int compat,
+                       int64_t sy, int64_t sz, const BndScalars<C>& k)
+{
+    const uint32_t m = info & kInfoMask;
+    const int K = (int)((info >> kInfoKShift) & 3u);
+    const unsigned sf = (compat && K == 3) ? (unsigned)N / 2u : (unsigned)N;  // lrs_3.cl:70
+    C a0[3], b0[3], g[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (q >= K) break;
+        const int mat = (int)((mats >> (10 * q)) & 1023u);
+        const C* ak = A + mat * N;
+        const C* bk = B + mat * N;
+        a0[q] = ak[0];
+        b0[q] = bk[0];
+        C gn = 0;
+        for (int ii = 1; ii < N; ++ii) {
+            const unsigned slot = ((ixb + (unsigned)ii) % sf) + (unsigned)q * sf;
+            gn += bk[ii] * rin[slot * rs + e] - ak[ii] * rout[slot * rs + e];
+        }
+        g[q] = gn;
+    }
+    const C l = k.lam_b, l2 = k.l2b;
+    const C pc = ld<S, C>(cur, i);
+    const C pd1 = ld<S, C>(prev, i);
+    C v;
+    if (info & kInfoNoUpdate) {
+        v = leapfrog_value<S, C>(cur, prev, i, sy, sz, k.l2, k.c2);
+    } else {
+        const int64_t st3[3] = {1, sy, sz};
+        C acc = 0;
+        bool first = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t plus = 1u << (2 * d), minus = 2u << (2 * d);
+            if (m & plus) {
+                const C tt = (C)2.0 * ld<S, C>(cur, i - st3[d]);
+                acc = first ? tt : acc + tt;
+            } else if (m & minus) {
+                const C tt = (C)2.0 * ld<S, C>(cur, i + st3[d]);
+                acc = first ? tt : acc + tt;
+            } else {
+                const C up = ld<S, C>(cur, i + st3[d]);
+                acc = first ? up : acc + up;
+                acc = acc + ld<S, C>(cur, i - st3[d]);
+            }
+            first = false;
+        }
+        C num, den;
+        if (K == 1) {
+            num = l2 * acc + k.c2b * pc;
+            num = num + l2 / b0[0] * g[0];
+            num = num + (l * a0[0] / b0[0] - (C)1.0) * pd1;
+            den = (C)1.0 + l * a0[0] / b0[0];
+        } else {
+            const int U = (compat && K == 3) ? 2 : K;  // lrs_3.cl:96-174 use materials 0 and 1
+            C G = g[0] / b0[0];
+            C Asum = l * a0[0] / b0[0];
+            den = (C)1.0 + l * a0[0] / b0[0];
+            for (int q = 1; q < U; ++q) {
+                G = G + g[q] / b0[q];
+                Asum = Asum + l * a0[q] / b0[q];
+                den = den + l * a0[q] / b0[q];
+            }
+            num = l2 * acc + l2 * G;
+            num = num + k.c2b * pc;
+            num = num + (Asum - (C)1.0) * pd1;
+        }
+        v = num / den;
+    }
+    v = rt<S, C>(v);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (q >= K) break;
+        const C xn = a0[q] / (l * b0[q]) * (v - pd1) - g[q] / b0[q];
+        const unsigned w = ixb + (unsigned)q * sf;
+        rin[w * rs + e] = xn;
+        rout[w * rs + e] = (C)1.0 / a0[q] * (b0[q] * xn + g[q]);
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned ring_index(int64_t t, int N)
+{
+    // simulation.hpp:365 decrements the ring index before every step's LRS
+    // launches, starting from 0: step t (0-based) uses (-(t+1)) mod N.
+    const int64_t r = (t + 1) % N;
+    return (unsigned)((N - r) % N);
+}
+
+template <typename S, typename C>
+struct BndCtx {
+    S* out;
+    const S* cur;
+    const S* prev;
+    const RoomDev* rooms;
+    const int32_t* e_cell;
+    const uint32_t* e_info;
+    const uint32_t* e_mat;
+    C* rin;
+    C* rout;
+    int64_t rs;
+    const C* A;
+    const C* B;
+    int N;
+    int compat;
+    BndScalars<C> k;
+    // special
+    const int32_t* sp_cell;
+    const int32_t* sp_room;
+    const int32_t* sp_bidx;
+    const int32_t* sp_src_off;
+    const int32_t* sp_src;
+    const int32_t* sp_mic_off;
+    const int32_t* sp_mic;
+    const float* src;
+    int64_t src_cap;
+    const int64_t* src_len;
+    float* traces;
+    int64_t trace_cap;
+    C scale, inv_scale;
+};
+
+template <typename S, typename C>
+__device__ __forceinline__ void boundary_entry(const BndCtx<S, C>& c, int e, unsigned ixb, bool skip_special)
+{
+    const uint32_t info = c.e_info[e];
+    if (skip_special && (info & kInfoSpecial)) return;
+    const RoomDev r = c.rooms[info >> kInfoRoomShift];
+    const int64_t i = c.e_cell[e];
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    const C v = bnd_value<S, C>(e, i, info, c.e_mat[e], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+                                c.compat, sy, sz, c.k);
+    st<S, C>(c.out, i, v);
+}
+
+// mode 1: full update (boundary or leapfrog value) + sources + receivers
+// mode 2: add sources to the stored value + receivers (after a PML pass)
+template <typename S, typename C>
+__device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsigned ixb, int64_t t, int mode)
+{
+    const int64_t i = c.sp_cell[j];
+    const int b = c.sp_bidx[j];
+    C v;
+    if (mode == 1 && b >= 0) {
+        const uint32_t info = c.e_info[b];
+        const RoomDev r = c.rooms[info >> kInfoRoomShift];
+        v = bnd_value<S, C>(b, i, info, c.e_mat[b], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+                            c.compat, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k);
+    } else if (mode == 1 && b == -1) {
+        const RoomDev r = c.rooms[c.sp_room[j]];
+        v = rt<S, C>(leapfrog_value<S, C>(c.cur, c.prev, i, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k.l2, c.k.c2));
+    } else {
+        // edge planes (b == -2) and add-only mode keep the stored value
+        v = ld<S, C>(c.out, i);
+    }
+    for (int q = c.sp_src_off[j]; q < c.sp_src_off[j + 1]; ++q) {
+        const int s = c.sp_src[q];
+        const C sv = t < c.src_len[s] ? (C)c.src[(int64_t)s * c.src_cap + t] : (C)0;
+        v = v + sv * c.inv_scale;
+    }
+    st<S, C>(c.out, i, v);
+    const C stored = ld<S, C>(c.out, i);
+    for (int q = c.sp_mic_off[j]; q < c.sp_mic_off[j + 1]; ++q) {
+        const int m = c.sp_mic[q];
+        c.traces[(int64_t)m * c.trace_cap + t] = (float)(stored * c.scale);
+    }
+}
+
+template <typename S, typename C>
+__host__ BndCtx<S, C> make_ctx(const StepArgs& a)
+{
+    BndCtx<S, C> c;
+    c.out = (S*)a.out;
+    c.cur = (const S*)a.cur;
+    c.prev = (const S*)a.prev;
+    c.rooms = a.rooms;
+    c.e_cell = a.e_cell;
+    c.e_info = a.e_info;
+    c.e_mat = a.e_mat;
+    c.rin = (C*)a.ring_in;
+    c.rout = (C*)a.ring_out;
+    c.rs = a.nb;
+    c.A = (const C*)a.coef_a;
+    c.B = (const C*)a.coef_b;
+    c.N = a.ncoef;
+    c.compat = a.compat;
+    c.k.lam_b = (C)a.lam_b;
+    c.k.l2b = (C)a.l2b;
+    c.k.c2b = (C)a.c2b;
+    c.k.l2 = (C)a.l2;
+    c.k.c2 = (C)a.c2;
+    c.sp_cell = a.sp_cell;
+    c.sp_room = a.sp_room;
+    c.sp_bidx = a.sp_bidx;
+    c.sp_src_off = a.sp_src_off;
+    c.sp_src = a.sp_src;
+    c.sp_mic_off = a.sp_mic_off;
+    c.sp_mic = a.sp_mic;
+    c.src = a.src;
+    c.src_cap = a.src_cap;
+    c.src_len = a.src_len;
+    c.traces = a.traces;
+    c.trace_cap = a.trace_cap;
+    c.scale = (C)a.scale;
+    c.inv_scale = (C)(1.0 / a.scale);
+    return c;
+}
+
+constexpr int kBndThreads = 256;
+
+template <typename S, typename C>
+__global__ __launch_bounds__(kBndThreads) void k_boundary(BndCtx<S, C> c, int nb, int nblk_b, int nsp,
+                                                          const int64_t* __restrict__ d_t, int t_rel,
+                                                          int skip_special, int special_mode)
+{
+    const int64_t t = *d_t + t_rel;
+    const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
+    if ((int)blockIdx.x < nblk_b) {
+        const int e = blockIdx.x * kBndThreads + threadIdx.x;
+        if (e < nb) boundary_entry<S, C>(c, e, ixb, skip_special != 0);
+    } else {
+        const int j = (blockIdx.x - nblk_b) * kBndThreads + threadIdx.x;
+        if (j < nsp && special_mode != 0) special_entry<S, C>(c, j, ixb, t, special_mode);
+    }
+}
+
+// ---------------------------------------------------------------- fused step
+// One launch per step: each workgroup updates its tile's interior, waits on
+// a workgroup barrier, then overwrites the tile's boundary cells and
+// finishes its source/receiver cells.  Entries are sorted by tile so
+// tile_e_off/tile_sp_off (indexed like the tile table) delimit them.
+template <typename S, int TX, int TY, int TZ>
+__global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>::T> c,
+                                                   const int4* __restrict__ tiles,
+                                                   const int32_t* __restrict__ tile_e_off,
+                                                   const int32_t* __restrict__ tile_e_perm,
+                                                   const int32_t* __restrict__ tile_sp_off,
+                                                   const int64_t* __restrict__ d_t, int t_rel)
+{
+    using C = typename Compute<S>::T;
+    const int4 t4 = tiles[blockIdx.x];
+    if (t4.x < 0) return;  // uniform per workgroup
+    const RoomDev r = c.rooms[t4.x];
+    const int x = t4.y + (int)(threadIdx.x % TX);
+    const int y = t4.z + (int)(threadIdx.x / TX);
+    interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2);
+    const int eb = tile_e_off[blockIdx.x], ee = tile_e_off[blockIdx.x + 1];
+    const int sb = tile_sp_off[blockIdx.x], se = tile_sp_off[blockIdx.x + 1];
+    if (eb == ee && sb == se) return;  // uniform per workgroup
+    __syncthreads();
+    const int64_t t = *d_t + t_rel;
+    const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
+    for (int k = eb + (int)threadIdx.x; k < ee; k += TX * TY) boundary_entry<S, C>(c, tile_e_perm[k], ixb, true);
+    for (int j = sb + (int)threadIdx.x; j < se; j += TX * TY) special_entry<S, C>(c, j, ixb, t, 1);
+}
+
+__global__ void k_advance_t(int64_t* d_t, int64_t k)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d_t += k;
+}
+
+// ---------------------------------------------------------------- PML, physical
+// Grote-Sim PML on the slab boxes.  psi is cell-centred at half steps; phi_x
+// lives at x+1/2 and is stored at index x (likewise y, z).  Pressure part
+// runs after the interior launch, before the boundary/special launch.
+template <typename S>
+__global__ void k_pml_p(PmlArgs a)
+{
+    using C = typename Compute<S>::T;
+    const int32_t* bx = a.boxes + 7 * blockIdx.y;
+    const int room = bx[0];
+    const int wx = bx[2] - bx[1], wy = bx[4] - bx[3], wz = bx[6] - bx[5];
+    const int64_t nbox = (int64_t)wx * wy * wz;
+    const RoomDev r = a.rooms[room];
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    S* out = (S*)a.out;
+    const S* cur = (const S*)a.cur;
+    const S* prev = (const S*)a.prev;
+    C* phx = (C*)a.phi[0];
+    C* phy = (C*)a.phi[1];
+    C* phz = (C*)a.phi[2];
+    C* psi = (C*)a.psi;
+    const C* zx = (const C*)a.z[0] + a.axis_off[3 * room + 0];
+    const C* zy = (const C*)a.z[1] + a.axis_off[3 * room + 1];
+    const C* zz = (const C*)a.z[2] + a.axis_off[3 * room + 2];
+    const C l2 = (C)a.l2, dx = (C)a.dx, dt = (C)a.dt;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += (int64_t)gridDim.x * blockDim.x) {
+        const int x = bx[1] + (int)(q % wx);
+        const int y = bx[3] + (int)((q / wx) % wy);
+        const int z = bx[5] + (int)(q / ((int64_t)wx * wy));
+        const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
+        const C sx = zx[x], syy = zy[y], szz = zz[z];
+        const C s1 = sx + syy + szz;
+        const C s2 = sx * syy + syy * szz + szz * sx;
+        const C s3 = sx * syy * szz;
+        const C pc = ld<S, C>(cur, i);
+        const C pm = ld<S, C>(prev, i);
+        C sum = ld<S, C>(cur, i + 1) + ld<S, C>(cur, i - 1);
+        sum = sum + ld<S, C>(cur, i + sy);
+        sum = sum + ld<S, C>(cur, i - sy);
+        sum = sum + ld<S, C>(cur, i + sz);
+        sum = sum + ld<S, C>(cur, i - sz);
+        const C dv = (phx[i] - phx[i - 1]) + (phy[i] - phy[i - sy]) + (phz[i] - phz[i - sz]);
+        const C psn = psi[i] + dt * pc;
+        C num = l2 * (sum - (C)6.0 * pc);
+        num = num + (dt * dt / dx) * dv;
+        num = num - (dt * dt) * s3 * (C)0.5 * (psn + psi[i]);
+        num = num - (dt * dt) * s2 * pc;
+        num = num + (C)2.0 * pc - pm;
+        num = num + (C)0.5 * dt * s1 * pm;
+        st<S, C>(out, i, num / ((C)1.0 + (C)0.5 * dt * s1));
+        psi[i] = psn;
+    }
+}
+
+// Auxiliary phi update; runs after the step's pressure is final.
+template <typename S>
+__global__ void k_pml_phi(PmlArgs a)
+{
+    using C = typename Compute<S>::T;
+    const int32_t* bx = a.boxes + 7 * blockIdx.y;
+    const int room = bx[0];
+    const int wx = bx[2] - bx[1], wy = bx[4] - bx[3], wz = bx[6] - bx[5];
+    const int64_t nbox = (int64_t)wx * wy * wz;
+    const RoomDev r = a.rooms[room];
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    const S* p1 = (const S*)a.out;
+    const S* p = (const S*)a.cur;
+    C* phx = (C*)a.phi[0];
+    C* phy = (C*)a.phi[1];
+    C* phz = (C*)a.phi[2];
+    const C* psi = (const C*)a.psi;
+    const C* zx = (const C*)a.z[0] + a.axis_off[3 * room + 0];
+    const C* zy = (const C*)a.z[1] + a.axis_off[3 * room + 1];
+    const C* zz = (const C*)a.z[2] + a.axis_off[3 * room + 2];
+    const C* hx = (const C*)a.zh[0] + a.axis_off[3 * room + 0];
+    const C* hy = (const C*)a.zh[1] + a.axis_off[3 * room + 1];
+    const C* hz = (const C*)a.zh[2] + a.axis_off[3 * room + 2];
+    const C c = (C)a.c, dx = (C)a.dx, dt = (C)a.dt;
+    const C cc = c * c, idt = (C)1.0 / dt, idx = (C)1.0 / dx;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += (int64_t)gridDim.x * blockDim.x) {
+        const int x = bx[1] + (int)(q % wx);
+        const int y = bx[3] + (int)((q / wx) % wy);
+        const int z = bx[5] + (int)(q / ((int64_t)wx * wy));
+        const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
+        const C p1c = ld<S, C>(p1, i), pcc = ld<S, C>(p, i);
+        {
+            const C h = hx[x];
+            const C g2 = zy[y] + zz[z] - h, g3 = zy[y] * zz[z];
+            const C dn = (ld<S, C>(p1, i + 1) - p1c) * idx, dold = (ld<S, C>(p, i + 1) - pcc) * idx;
+            const C dps = (psi[i + 1] - psi[i]) * idx;
+            phx[i] = ((idt - (C)0.5 * h) * phx[i] + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
+        }
+        {
+            const C h = hy[y];
+            const C g2 = zx[x] + zz[z] - h, g3 = zx[x] * zz[z];
+            const C dn = (ld<S, C>(p1, i + sy) - p1c) * idx, dold = (ld<S, C>(p, i + sy) - pcc) * idx;
+            const C dps = (psi[i + sy] - psi[i]) * idx;
+            phy[i] = ((idt - (C)0.5 * h) * phy[i] + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
+        }
+        {
+            const C h = hz[z];
+            const C g2 = zx[x] + zy[y] - h, g3 = zx[x] * zy[y];
+            const C dn = (ld<S, C>(p1, i + sz) - p1c) * idx, dold = (ld<S, C>(p, i + sz) - pcc) * idx;
+            const C dps = (psi[i + sz] - psi[i]) * idx;
+            phz[i] = ((idt - (C)0.5 * h) * phz[i] + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- PML, compat
+// pml.cl:154-265 over 1 < x < n-2 (likewise y, z).  The reference reads p and
+// psiph at +1 neighbours while other work-items write them; here the three
+// parts run as ordered phases (psi, p, fi), each reading the completed
+// previous phase.  Only room 0 (compat is single-room).
+template <typename S, int PHASE>
+__global__ void k_pml_compat(PmlArgs a)
+{
+    using C = typename Compute<S>::T;
+    const RoomDev r = a.rooms[0];
+    const int nx = r.nx, ny = r.ny, nz = r.nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int64_t wx = nx - 4, wy = ny - 4, wz = nz - 4;
+    if (wx <= 0 || wy <= 0 || wz <= 0) return;
+    const int64_t total = wx * wy * wz;
+    S* p = (S*)a.out;
+    const S* pm1 = (const S*)a.cur;
+    const S* pm2 = (const S*)a.prev;
+    C* fx = (C*)a.phi[0];
+    C* fy = (C*)a.phi[1];
+    C* fz = (C*)a.phi[2];
+    const C* fxm1 = (const C*)a.phi_old[0];
+    const C* fym1 = (const C*)a.phi_old[1];
+    const C* fzm1 = (const C*)a.phi_old[2];
+    C* psiph = (C*)a.psi;
+    const C* psimh = (const C*)a.psi_old;
+    const C* kx = (const C*)a.z[0];
+    const C* ky = (const C*)a.z[1];
+    const C* kz = (const C*)a.z[2];
+    const C c = (C)a.c, dx = (C)a.dx, dt = (C)a.dt;
+    const int L = a.width;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int x = 2 + (int)(q % wx);
+        const int y = 2 + (int)((q / wx) % wy);
+        const int z = 2 + (int)(q / (wx * wy));
+        const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
+        if (PHASE == 0) {
+            psiph[i] = dt * ld<S, C>(pm1, i) + psimh[i];
+        } else if (PHASE == 1) {
+            auto P1 = [&](int64_t j) { return ld<S, C>(pm1, j); };
+            C lap = P1(i + 1) + P1(i - 1) + P1(i + sy) + P1(i - sy) + P1(i + sz) + P1(i - sz) - (C)6.0 * P1(i);
+            auto av = [&](const C* f, int64_t i0, int64_t da, int64_t db) {
+                return (C)0.25 * (f[i0] + f[i0 + db] + f[i0 + da] + f[i0 + da + db]);
+            };
+            C dv = av(fxm1, i + 1, sy, sz) - av(fxm1, i, sy, sz) + av(fym1, i + sy, 1, sz) - av(fym1, i, 1, sz)
+                   + av(fzm1, i + sz, 1, sy) - av(fzm1, i, 1, sy);
+            const C t1 = c * c / (dx * dx) * lap;
+            const C t2 = (C)1.0 / dx * dv;
+            const C t3 = kx[x] * ky[y] * kz[z] * (C)0.5 * (psiph[i] + psimh[i]);
+            const C t4 = (C)1.0 / (dt * dt) * (ld<S, C>(pm2, i) - (C)2.0 * P1(i));
+            const C t5 = (kx[x] * ky[y] + ky[y] * kz[z] + kz[z] * kx[x]) * P1(i);
+            const C t6 = (kx[x] + ky[y] + kz[z]) / ((C)2.0 * dt) * ld<S, C>(pm2, i);
+            const C num = t1 + t2 - t3 - t4 - t5 + t6;
+            st<S, C>(p, i, num / ((C)1.0 / (dt * dt) + (kx[x] + ky[y] + kz[z]) / ((C)2.0 * dt)));
+        } else {
+            auto avs = [&](const S* f, int64_t i0, int64_t da, int64_t db) {
+                return (C)0.25 * (ld<S, C>(f, i0) + ld<S, C>(f, i0 + db) + ld<S, C>(f, i0 + da) + ld<S, C>(f, i0 + da + db));
+            };
+            auto avc = [&](const C* f, int64_t i0, int64_t da, int64_t db) {
+                return (C)0.25 * (f[i0] + f[i0 + db] + f[i0 + da] + f[i0 + da + db]);
+            };
+            const C kax = (C)0.5 * (kx[x] + kx[x + 1]);
+            const C kay = (C)0.5 * (ky[y] + ky[y + 1]);
+            const C kaz = (C)0.5 * (kz[z] + kz[z + 1]);
+            if (x < L || x > nx - L - 1) {
+                const C d = (C)0.5 / dx * (avs(p, i + 1, sy, sz) - avs(p, i, sy, sz) + avs(pm1, i + 1, sy, sz) - avs(p, i, sy, sz));
+                const C dp = (C)1.0 / dx * (avc(psiph, i + 1, sy, sz) - avc(psiph, i, sy, sz));
+                fx[i] = (-(C)0.5 * kax * fxm1[i] + (kay + kaz - kax) * d + (kay * kaz) * dp + (C)1.0 / dt * (fxm1[i]))
+                        / ((C)1.0 / dt + kax * (C)0.5);
+            }
+            if (y < L || y > ny - L - 1) {
+                const C d = (C)0.5 / dx * (avs(p, i + sy, 1, sz) - avs(p, i, 1, sz) + avs(pm1, i + sy, 1, sz) - avs(p, i, 1, sz));
+                const C dp = (C)1.0 / dx * (avc(psiph, i + sy, 1, sz) - avc(psiph, i, 1, sz));
+                const C kxy = (C)0.5 * (kx[y] + kx[y + 1]);  // ksi_avg(ksi_x, y), pml.cl:247
+                fy[i] = (-(C)0.5 * kay * fym1[i] + (kax + kaz - kay) * d + (kxy * kaz) * dp + (C)1.0 / dt * (fym1[i]))
+                        / ((C)1.0 / dt + kay * (C)0.5);
+            }
+            if (z < L || z > nz - L - 1) {
+                const C d = (C)0.5 / dx * (avs(p, i + sz, 1, sy) - avs(p, i, 1, sy) + avs(pm1, i + sz, 1, sy) - avs(p, i, 1, sy));
+                const C dp = (C)1.0 / dx * (avc(psiph, i + sz, 1, sy) - avc(psiph, i, 1, sy));
+                fz[i] = (-(C)0.5 * kaz * fzm1[i] + (kax + kay - kaz) * d + (kax * kay) * dp + (C)1.0 / dt * (fzm1[i]))
+                        / ((C)1.0 / dt + kaz * (C)0.5);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- utilities
+__global__ void k_fill_zero(uint32_t* p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+template <typename S>
+__global__ void k_to_f32(const S* __restrict__ src, float* __restrict__ dst, int64_t n, double scale)
+{
+    using C = typename Compute<S>::T;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)((C)src[i] * (C)scale);
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename S, int TX, int TY, int TZ>
+static hipError_t interior_t(const StepArgs& a, hipStream_t s)
+{
+    using C = typename Compute<S>::T;
+    hipLaunchKernelGGL((k_interior<S, TX, TY, TZ>), dim3(a.ntiles), dim3(TX * TY), 0, s, (S*)a.out,
+                       (const S*)a.cur, (const S*)a.prev, a.tiles, a.rooms, (C)a.l2, (C)a.c2);
+    return hipGetLastError();
+}
+
+template <typename S, int TX, int TY, int TZ>
+static hipError_t fused_t(const StepArgs& a, const int32_t* teo, const int32_t* tep, const int32_t* tso,
+                          hipStream_t s)
+{
+    using C = typename Compute<S>::T;
+    BndCtx<S, C> c = make_ctx<S, C>(a);
+    hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.ntiles), dim3(TX * TY), 0, s, c, a.tiles, teo, tep, tso, a.d_t,
+                       a.t_rel);
+    return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t interior_s(const StepArgs& a, hipStream_t s)
+{
+    switch (a.tile) {
+    case 0: return interior_t<S, 64, 4, 16>(a, s);
+    case 1: return interior_t<S, 64, 4, 32>(a, s);
+    case 2: return interior_t<S, 64, 8, 16>(a, s);
+    case 3: return interior_t<S, 64, 2, 32>(a, s);
+    case 4: return interior_t<S, 128, 2, 16>(a, s);
+    case 5: return interior_t<S, 64, 4, 8>(a, s);
+    case 6: return interior_t<S, 256, 1, 16>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename S>
+static hipError_t fused_s(const StepArgs& a, const int32_t* teo, const int32_t* tep, const int32_t* tso,
+                          hipStream_t s)
+{
+    switch (a.tile) {
+    case 0: return fused_t<S, 64, 4, 16>(a, teo, tep, tso, s);
+    case 1: return fused_t<S, 64, 4, 32>(a, teo, tep, tso, s);
+    case 2: return fused_t<S, 64, 8, 16>(a, teo, tep, tso, s);
+    case 3: return fused_t<S, 64, 2, 32>(a, teo, tep, tso, s);
+    case 4: return fused_t<S, 128, 2, 16>(a, teo, tep, tso, s);
+    case 5: return fused_t<S, 64, 4, 8>(a, teo, tep, tso, s);
+    case 6: return fused_t<S, 256, 1, 16>(a, teo, tep, tso, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_interior(const StepArgs& a, hipStream_t s)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    switch (a.prec) {
+    case Prec::F32: return interior_s<float>(a, s);
+    case Prec::F64: return interior_s<double>(a, s);
+    case Prec::F16: return interior_s<_Float16>(a, s);
+    case Prec::BF16: return interior_s<__bf16>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fused(const StepArgs& a, const int32_t* teo, const int32_t* tep, const int32_t* tso, hipStream_t s)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    switch (a.prec) {
+    case Prec::F32: return fused_s<float>(a, teo, tep, tso, s);
+    case Prec::F64: return fused_s<double>(a, teo, tep, tso, s);
+    case Prec::F16: return fused_s<_Float16>(a, teo, tep, tso, s);
+    case Prec::BF16: return fused_s<__bf16>(a, teo, tep, tso, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename S>
+static hipError_t boundary_s(const StepArgs& a, int skip_special, int special_mode, hipStream_t s)
+{
+    using C = typename Compute<S>::T;
+    const int nblk_b = (a.nb + kBndThreads - 1) / kBndThreads;
+    const int nsp = special_mode ? a.nsp : 0;
+    const int nblk_s = (nsp + kBndThreads - 1) / kBndThreads;
+    if (nblk_b + nblk_s == 0) return hipSuccess;
+    BndCtx<S, C> c = make_ctx<S, C>(a);
+    hipLaunchKernelGGL((k_boundary<S, C>), dim3(nblk_b + nblk_s), dim3(kBndThreads), 0, s, c, a.nb, nblk_b, nsp,
+                       a.d_t, a.t_rel, skip_special, special_mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary(const StepArgs& a, int skip_special, int special_mode, hipStream_t s)
+{
+    switch (a.prec) {
+    case Prec::F32: return boundary_s<float>(a, skip_special, special_mode, s);
+    case Prec::F64: return boundary_s<double>(a, skip_special, special_mode, s);
+    case Prec::F16: return boundary_s<_Float16>(a, skip_special, special_mode, s);
+    case Prec::BF16: return boundary_s<__bf16>(a, skip_special, special_mode, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_advance_t(int64_t* d_t, int64_t k, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, s, d_t, k);
+    return hipGetLastError();
+}
+
+static dim3 pml_grid(const PmlArgs& a)
+{
+    int64_t blocks = (a.box_cells_max + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    return dim3((unsigned)blocks, (unsigned)a.nbox);
+}
+
+hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return hipSuccess;
+    switch (a.prec) {
+    case Prec::F32: hipLaunchKernelGGL(k_pml_p<float>, pml_grid(a), dim3(256), 0, s, a); break;
+    case Prec::F64: hipLaunchKernelGGL(k_pml_p<double>, pml_grid(a), dim3(256), 0, s, a); break;
+    case Prec::F16: hipLaunchKernelGGL(k_pml_p<_Float16>, pml_grid(a), dim3(256), 0, s, a); break;
+    case Prec::BF16: hipLaunchKernelGGL(k_pml_p<__bf16>, pml_grid(a), dim3(256), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pml_phys_phi(const PmlArgs& a, hipStream_t s)
+{
+    if (a.nbox <= 0) return hipSuccess;
+    switch (a.prec) {
+    case Prec::F32: hipLaunchKernelGGL(k_pml_phi<float>, pml_grid(a), dim3(256), 0, s, a); break;
+    case Prec::F64: hipLaunchKernelGGL(k_pml_phi<double>, pml_grid(a), dim3(256), 0, s, a); break;
+    case Prec::F16: hipLaunchKernelGGL(k_pml_phi<_Float16>, pml_grid(a), dim3(256), 0, s, a); break;
+    case Prec::BF16: hipLaunchKernelGGL(k_pml_phi<__bf16>, pml_grid(a), dim3(256), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename S>
+static hipError_t pml_compat_s(const PmlArgs& a, hipStream_t s)
+{
+    const dim3 g(2048), b(256);
+    hipLaunchKernelGGL((k_pml_compat<S, 0>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_pml_compat<S, 1>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_pml_compat<S, 2>), g, b, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pml_compat(const PmlArgs& a, hipStream_t s)
+{
+    switch (a.prec) {
+    case Prec::F32: return pml_compat_s<float>(a, s);
+    case Prec::F64: return pml_compat_s<double>(a, s);
+    case Prec::F16: return pml_compat_s<_Float16>(a, s);
+    case Prec::BF16: return pml_compat_s<__bf16>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s)
+{
+    const int64_t n = bytes / 4;
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const void* src, Prec prec, float* dst, int64_t n, double scale, hipStream_t s)
+{
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) return hipSuccess;
+    switch (prec) {
+    case Prec::F32: hipLaunchKernelGGL(k_to_f32<float>, dim3((unsigned)blocks), dim3(256), 0, s, (const float*)src, dst, n, scale); break;
+    case Prec::F64: hipLaunchKernelGGL(k_to_f32<double>, dim3((unsigned)blocks), dim3(256), 0, s, (const double*)src, dst, n, scale); break;
+    case Prec::F16: hipLaunchKernelGGL(k_to_f32<_Float16>, dim3((unsigned)blocks), dim3(256), 0, s, (const _Float16*)src, dst, n, scale); break;
+    case Prec::BF16: hipLaunchKernelGGL(k_to_f32<__bf16>, dim3((unsigned)blocks), dim3(256), 0, s, (const __bf16*)src, dst, n, scale); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace afx

@@ -1,0 +1,130 @@
+"""Benchmark: FDTD room-acoustics step on one MI355X.
+
+A step is one leapfrog time step of the furnished room (256 x 192 x 128
+cells, 4 IIR wall materials, 12 seeded furniture boxes, 12-cell PML on one
+face, band-limited noise source, 16 receivers), FP32, physical boundary
+mode, run through the C ABI.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
+                  [--compat] [--no-cpu-baseline] [--tune]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "anc-field_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {"fp32": 12, "fp64": 24, "fp16": 6, "bf16": 6}
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """Time the CPU oracle (OpenMP C restatement) on the same room for a
+    bounded number of steps on this host's cores."""
+    import numpy as np
+    import orc
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    o = orc.from_workload(w, compat=False, threads=cores)
+    o.run(5)  # first touch
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.run(10)
+        steps += 10
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": round(w.cells * steps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{w.name} {w.dims[0]}x{w.dims[1]}x{w.dims[2]}, {steps} steps after 5 warm-up steps, "
+                      f"FP32 oracle (C, OpenMP) on {cores} host threads"}
+
+
+def traffic_for(name):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary, if one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_CELL))
+    ap.add_argument("--compat", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    from anc_field_py import _afx, rooms
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is optional for the measurement itself
+        sync = lambda: None  # noqa: E731
+
+    total = args.warmup + args.steps + args.profile_steps
+    w = rooms.furnished(steps=total)
+    sim = rooms.make_sim(w, _afx.Sim, compat=args.compat, precision=args.precision)
+    sim.run(args.warmup)
+    tile = sim.autotune(30) if args.tune else None
+    sync()
+    t0 = time.perf_counter()
+    sim.run(args.steps)
+    sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    gcells = w.cells * args.steps / elapsed / 1e9
+    st = sim.stats()
+    prof = sim.profile(args.profile_steps)
+    k_ms = prof.kernel_ms[0]
+    bpc = BYTES_PER_CELL[args.precision]
+    achieved = bpc * w.cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    kernel = "k_fused" if st.fused else "k_interior"
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(f"{w.name}_{args.precision}"),
+            "kernel": kernel, "kernel_ms": round(k_ms, 5),
+            "algorithmic_bytes_per_launch": bpc * w.cells}
+    out = {
+        "metric": "Gcell-updates/s",
+        "value": round(gcells, 2),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-storage/f32", "bf16": "bf16-storage/f32"}[args.precision],
+        "data": "synthetic (seeded room generator)",
+        "config": {"workload": f"furnished room {w.dims[0]}x{w.dims[1]}x{w.dims[2]} cells, dx {w.dx} m, "
+                               f"4 IIR materials, 12 boxes, 12-cell PML x+, 16 receivers",
+                   "mode": "compat" if args.compat else "physical",
+                   "boundary_cells": int(st.boundary), "fused_step_kernel": bool(st.fused),
+                   "tile": [st.tile_x, st.tile_y, st.tile_z], "autotuned": tile is not None,
+                   "profile": {"step_ms": round(prof.step_ms, 5),
+                               "kernel_ms": [round(x, 5) for x in prof.kernel_ms],
+                               "launches": list(prof.launches)}},
+        "roofline": roof,
+    }
+    sim.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,113 @@
+/*
+ * CPU oracle — TEST INFRASTRUCTURE ONLY (see orc.h).  Built by oracle/build.sh
+ * into oracle/liborc.so; loaded only by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg, never by the product library.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "orc.h"
+
+/* Decimal solid-neighbour patterns of lrs-data.hpp:24-51 mapped to the group
+ * ids of lrs-data.hpp:53-78 (faces 0-5, edges 6-17, corners 18-25). */
+static int orc_group_of(int pattern)
+{
+    static const int pat[26] = {
+        100000, 10000, 1000, 100, 10, 1,
+        101000, 100100, 100010, 100001, 11000, 10100, 10010, 10001, 1010, 1001, 110, 101,
+        101010, 101001, 100110, 100101, 11010, 11001, 10110, 10101,
+    };
+    for (int g = 0; g < 26; ++g)
+        if (pat[g] == pattern) return g;
+    return -1;
+}
+
+/* lrs-data.hpp:117-215: scan 1..n-2 on every axis, z outermost; AIR cells
+ * with solid neighbours are grouped.  Materials: faces take materials[group],
+ * edges/corners take the non-air neighbours in X_P,X_M,Y_P,Y_M,Z_P,Z_M order. */
+int64_t orc_classify(int32_t nx, int32_t ny, int32_t nz, const int32_t* v,
+                     int32_t* pos, int32_t* group, int64_t cap)
+{
+    const int64_t sx = nx, sxy = (int64_t)nx * ny;
+    int64_t count[26] = {0};
+    /* pass 1: counts per group (and the unphysical check) */
+    for (int64_t z = 1; z < nz - 1; ++z)
+        for (int64_t y = 1; y < ny - 1; ++y)
+            for (int64_t x = 1; x < nx - 1; ++x) {
+                const int64_t i = z * sxy + y * sx + x;
+                if (v[i] != -1) continue;
+                int pattern = 0;
+                if (v[i + 1] != -1) pattern += 100000;
+                if (v[i - 1] != -1) pattern += 10000;
+                if (v[i + sx] != -1) pattern += 1000;
+                if (v[i - sx] != -1) pattern += 100;
+                if (v[i + sxy] != -1) pattern += 10;
+                if (v[i - sxy] != -1) pattern += 1;
+                if (pattern == 0) continue;
+                int g = orc_group_of(pattern);
+                if (g < 0) {
+                    if (cap > 0) { pos[0] = (int32_t)x; pos[1] = (int32_t)y; pos[2] = (int32_t)z; }
+                    return -1;
+                }
+                count[g]++;
+            }
+    int64_t off[27];
+    off[0] = 0;
+    for (int g = 0; g < 26; ++g) off[g + 1] = off[g] + count[g];
+    if (off[26] > cap) return off[26];
+    int64_t fill[26];
+    for (int g = 0; g < 26; ++g) fill[g] = off[g];
+    for (int64_t z = 1; z < nz - 1; ++z)
+        for (int64_t y = 1; y < ny - 1; ++y)
+            for (int64_t x = 1; x < nx - 1; ++x) {
+                const int64_t i = z * sxy + y * sx + x;
+                if (v[i] != -1) continue;
+                int mats[6] = {-1, -1, -1, -1, -1, -1};
+                int pattern = 0;
+                if (v[i + 1] != -1) { mats[0] = v[i + 1]; pattern += 100000; }
+                if (v[i - 1] != -1) { mats[1] = v[i - 1]; pattern += 10000; }
+                if (v[i + sx] != -1) { mats[2] = v[i + sx]; pattern += 1000; }
+                if (v[i - sx] != -1) { mats[3] = v[i - sx]; pattern += 100; }
+                if (v[i + sxy] != -1) { mats[4] = v[i + sxy]; pattern += 10; }
+                if (v[i - sxy] != -1) { mats[5] = v[i - sxy]; pattern += 1; }
+                if (pattern == 0) continue;
+                int g = orc_group_of(pattern);
+                int64_t e = fill[g]++;
+                int32_t* q = pos + 6 * e;
+                q[0] = (int32_t)x; q[1] = (int32_t)y; q[2] = (int32_t)z;
+                q[3] = q[4] = q[5] = -1;
+                if (g <= 5) {
+                    q[3] = mats[g];
+                } else {
+                    int k = 0;
+                    for (int f = 0; f < 6; ++f)
+                        if (mats[f] != -1) q[3 + k++] = mats[f];
+                }
+                group[e] = g;
+            }
+    return off[26];
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+#define REAL float
+#define SFX _f32
+#include "orc_impl.h"
+#undef REAL
+#undef SFX
+
+#define REAL double
+#define SFX _f64
+#include "orc_impl.h"
+#undef REAL
+#undef SFX

@@ -1,0 +1,108 @@
+/*
+ * CPU oracle for the FDTD room-acoustics step — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithm (szarvas/anc-field) used
+ * by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py as
+ * the checker.  Nothing in the product library links, loads or calls it.
+ *
+ * Function -> reference text it restates (paths relative to the reference
+ * root) and how each is pinned:
+ *   orc_classify     anc-field/src/lrs-data.hpp:117-215 (scan), :218-330
+ *                    (pattern -> group), :24-78 (group ids).
+ *                    Pinned: brute-force pattern enumeration (tests), and the
+ *                    boundary-cell counts of examples/models/<name> (tests).
+ *   leapfrog         anc-field/src/leapfrog.cl:34-55.
+ *                    Pinned: discrete dispersion relation / rigid-box modes,
+ *                    free-field arrival, the ie224 reference-held traces.
+ *   lrs (26 groups)  lrs_1.cl:49-180, lrs_2.cl:49-216, lrs_3.cl:49-187,
+ *                    host sequence simulation.hpp:358-370 (ring index :365,
+ *                    lambda2 passed as `l` :368-369).
+ *                    Pinned: ie224 reference-held traces (ncoef=1 faces,
+ *                    edges, corners in compat mode), normal-incidence
+ *                    reflection R=(1-beta)/(1+beta) (physical), IIR ring
+ *                    recursion against scipy.signal.lfilter (tests).
+ *   pml (compat)     pml.cl:74-265, simulation.hpp:233-277, :373-382,
+ *                    :411-430.  Pinned only at the shipped state (kappa == 0,
+ *                    where it must reduce to the plain leapfrog); with a
+ *                    non-zero profile: parity unpinned.
+ *   pml (physical)   no reference counterpart (Grote & Sim 2010 PML for the
+ *                    scalar wave equation, the model pml.cl/sym_pml.py set
+ *                    out to discretise).  Pinned: reduces to leapfrog when
+ *                    zeta == 0; absorption against a rigid/Dirichlet wall.
+ *   source / mic     source.cl:32-40, mic.cl:32-41, simulation.hpp:384-392,
+ *                    positions simulation.hpp:322,338.
+ */
+#ifndef AFX_ORACLE_H
+#define AFX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* solid-face bits of a boundary cell, reference face order X_P..Z_M */
+#define ORC_XP 1u
+#define ORC_XM 2u
+#define ORC_YP 4u
+#define ORC_YM 8u
+#define ORC_ZP 16u
+#define ORC_ZM 32u
+
+typedef struct {
+    int32_t nx, ny, nz;
+    const int32_t* vox;     /* [nz][ny][nx], -1 air, else material id; may be NULL (all air) */
+    int32_t ncoef, nmat;
+    const float* a;         /* [nmat][ncoef] admittance numerator  */
+    const float* b;         /* [nmat][ncoef] admittance denominator */
+    float c, dx, dt;        /* lambda = dt*c/dx */
+    int32_t compat;         /* 1: reference quirks (boundary gets lambda^2, corner quirks) */
+    /* sources and receivers (cell coordinates) */
+    int32_t nsrc;
+    const int32_t* src_cell;  /* [nsrc][3] */
+    const float* src;         /* [nsrc][src_len] */
+    int64_t src_len;
+    int32_t nmic;
+    const int32_t* mic_cell;  /* [nmic][3] */
+    /* absorbing layer: 0 none, 1 compat pml.cl, 2 physical (Grote-Sim) */
+    int32_t pml_mode;
+    int32_t pml_width;
+    const float* ksi[3];      /* per-axis profiles: compat: kappa[nx|ny|nz];
+                                 physical: zeta at cell centres */
+    const float* ksi_h[3];    /* physical: zeta at +1/2 points */
+    const int32_t* pml_boxes; /* physical: [nbox][6] x0,x1,y0,y1,z0,z1 (half-open), disjoint */
+    int32_t pml_nbox;
+} orc_problem;
+
+/* Boundary classification restating lrs-data.hpp.  Writes up to `cap`
+ * entries in the reference's order (group 0..25, scan order within a group):
+ * pos[i*6 + {0..5}] = x, y, z, m0, m1, m2 (unused materials = -1), group[i].
+ * Returns the number of boundary cells, or -1 on an unphysical pattern
+ * (lrs-data.hpp:327), writing the offending cell into pos[0..2]. */
+int64_t orc_classify(int32_t nx, int32_t ny, int32_t nz, const int32_t* vox,
+                     int32_t* pos, int32_t* group, int64_t cap);
+
+typedef struct orc_sim_f32 orc_sim_f32;
+typedef struct orc_sim_f64 orc_sim_f64;
+
+orc_sim_f32* orc_create_f32(const orc_problem* pb);
+int          orc_run_f32(orc_sim_f32* s, int64_t steps, float* traces /*[nmic][steps] or NULL*/);
+/* which: 0 = newest p^{n+1}, 1 = p^n, 2 = p^{n-1}; writes nx*ny*nz values */
+void         orc_field_f32(const orc_sim_f32* s, int which, float* out);
+void         orc_destroy_f32(orc_sim_f32* s);
+int64_t      orc_nbound_f32(const orc_sim_f32* s);
+/* IIR state of the boundary cells in classify order: [nb][3*ncoef] each */
+void         orc_rings_f32(const orc_sim_f32* s, float* in, float* out);
+
+orc_sim_f64* orc_create_f64(const orc_problem* pb);
+int          orc_run_f64(orc_sim_f64* s, int64_t steps, double* traces);
+void         orc_field_f64(const orc_sim_f64* s, int which, double* out);
+void         orc_destroy_f64(orc_sim_f64* s);
+int64_t      orc_nbound_f64(const orc_sim_f64* s);
+void         orc_rings_f64(const orc_sim_f64* s, double* in, double* out);
+
+void         orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""HIP path against the CPU oracle, through the C ABI, on seeded inputs.
+
+FP32 and FP64 are compared bit for bit (both sides evaluate the reference's
+operations in the same order with IEEE arithmetic and no contraction).
+"""
+import numpy as np
+import pytest
+
+from anc_field_py import rooms
+import orc
+
+pytestmark = pytest.mark.gpu
+
+afx = pytest.importorskip("anc_field_py._afx")
+
+
+def _closed_room(seed, dims=(40, 36, 28), ncoef=3, nbox=3, nmat=3):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nx, ny, nz = dims
+    if ncoef == 3:
+        a, b = rooms.materials(rng, nmat)
+    else:
+        # frequency-dependent admittances of other orders: stable, passive-ish
+        a = rng.uniform(0.02, 0.3, (nmat, ncoef)).astype(np.float32) * np.array([1.0] + [0.3] * (ncoef - 1), np.float32)
+        b = np.zeros((nmat, ncoef), np.float32)
+        b[:, 0] = 1.0
+        if ncoef > 1:
+            b[:, 1:] = rng.uniform(-0.2, 0.2, (nmat, ncoef - 1))
+    face_mats = [int(rng.integers(0, nmat)) for _ in range(6)]
+    v = rooms.shoebox(nx, ny, nz, face_mats)
+    v = rooms.repair(rooms.add_boxes(v, rng, nbox, 4, 9, nmat))
+    return v, a, b, rng
+
+
+def _special_cells(v, rng):
+    """Source and receiver cells: interior, a wall face, the X_M_Y_M_Z_M
+    corner (1,1,1), an edge cell and an outer-plane cell."""
+    inner = rooms.interior_cells(v, rng, 3)
+    pos, grp = orc.classify(v)
+    face = pos[grp == 2][0, :3]
+    edge = pos[grp == 11][0, :3]
+    corner = np.array([1, 1, 1], np.int32)
+    assert v[1, 1, 1] == -1 and (grp[(pos[:, 0] == 1) & (pos[:, 1] == 1) & (pos[:, 2] == 1)] == 25).all()
+    plane = np.array([0, 5, 5], np.int32)
+    src_cells = np.stack([inner[0], face, corner, inner[0], plane])  # two sources share inner[0]
+    mic_cells = np.stack([inner[1], inner[2], face, edge, corner, inner[0], plane])
+    return src_cells, mic_cells
+
+
+def _run_pair(v, a, b, compat, src_cells, src, mic_cells, steps, precision="fp32", pml=None, lib_kw=None,
+              split_runs=None):
+    dx = 0.02
+    dt = rooms.courant_dt(dx) if not compat else 3.125e-05
+    c = rooms.C_AIR if not compat else 340.0
+    lib_kw = dict(lib_kw or {})
+    s = afx.Sim(c=c, dx=dx, dt=dt, res=dx, precision=precision, compat=compat, **lib_kw)
+    s.set_voxels(v)
+    s.set_materials(a, b)
+    pml_mode = 0
+    if pml is not None:
+        width, faces, profile = pml
+        s.set_pml(width, faces, profile)
+        pml_mode = 1 if compat else 2
+    for cell, sig in zip(src_cells, src):
+        s.add_source_cell(cell, sig)
+    for cell in mic_cells:
+        s.add_receiver_cell(cell)
+    for k in (split_runs or [steps]):
+        s.run(k)
+    got = s.receivers()
+    field = s.pressure(0)
+    o = orc.Oracle(v, a, b, c, dx, dt, compat, src_cells, src, mic_cells,
+                   "f64" if precision == "fp64" else "f32", pml_mode,
+                   pml[0] if pml else 0, pml[1] if pml else 0, pml[2] if pml else None)
+    o.run(steps)
+    want = o.receivers()
+    wfield = o.field(0)
+    s.close()
+    return got, want, field, wfield
+
+
+def _assert_bitexact(got, want, what):
+    got = np.asarray(got)
+    want = np.asarray(want).astype(got.dtype)
+    if not np.array_equal(got, want):
+        d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+        idx = np.unravel_index(np.argmax(d), d.shape)
+        rel = np.linalg.norm(d) / max(np.linalg.norm(want.astype(np.float64)), 1e-300)
+        raise AssertionError(f"{what}: {np.count_nonzero(d)} values differ, max |d| {d.max():.3e} at {idx}, rel L2 {rel:.3e}")
+
+
+def _case(name):
+    print(f"\n[case] {name}", flush=True)
+
+
+@pytest.mark.parametrize("compat", [True, False])
+@pytest.mark.parametrize("ncoef", [1, 3, 4])
+def test_room_bitexact(compat, ncoef):
+    _case(f"room_bitexact compat={compat} ncoef={ncoef}")
+    v, a, b, rng = _closed_room(10 + ncoef, ncoef=ncoef)
+    sc, mc = _special_cells(v, rng)
+    steps = 300
+    dt = 3.125e-05 if compat else rooms.courant_dt(0.02)
+    src = np.stack([rooms.ricker(steps, dt, 1000.0), rooms.ricker(steps, dt, 1500.0, amp=0.5),
+                    rooms.ricker(steps, dt, 800.0, amp=0.7), rooms.ricker(steps // 2, dt, 2000.0),
+                    rooms.ricker(steps, dt, 1200.0)])
+    got, want, f, wf = _run_pair(v, a, b, compat, sc, src, mc, steps)
+    assert np.abs(want).max() > 0
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+@pytest.mark.parametrize("flags", [dict(fused=False), dict(graph=False), dict(fused=False, graph=False)])
+def test_launch_variants_bitexact(flags):
+    _case(f"launch variants {flags}")
+    v, a, b, rng = _closed_room(21)
+    sc, mc = _special_cells(v, rng)
+    steps = 200
+    dt = rooms.courant_dt(0.02)
+    src = np.stack([rooms.ricker(steps, dt)] * len(sc))
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, lib_kw=flags)
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+def test_resumable_runs():
+    _case("resumable runs 37+50+13")
+    v, a, b, rng = _closed_room(22)
+    sc, mc = _special_cells(v, rng)
+    dt = rooms.courant_dt(0.02)
+    src = np.stack([rooms.ricker(100, dt)] * len(sc))
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, 100, split_runs=[37, 50, 13])
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+def test_physical_pml_bitexact():
+    _case("physical PML x+ and y- faces")
+    v, a, b, rng = _closed_room(23, dims=(48, 40, 30))
+    v[:, :, -1] = -1   # open x+ face
+    v[:, 0, :] = -1    # open y- face
+    v = rooms.repair(v)
+    sc = rooms.interior_cells(v, rng, 1)
+    mc = rooms.interior_cells(v, rng, 4)
+    steps = 250
+    dt = rooms.courant_dt(0.02)
+    src = rooms.ricker(steps, dt)[None]
+    prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, pml=(6, 2 | 4, prof))
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+@pytest.mark.parametrize("profile", [None, "shipped"])
+def test_compat_pml_bitexact(profile):
+    _case(f"compat PML profile={profile}")
+    v, a, b, rng = _closed_room(24, dims=(36, 32, 28))
+    sc = rooms.interior_cells(v, rng, 1)
+    mc = rooms.interior_cells(v, rng, 3)
+    steps = 120
+    src = rooms.ricker(steps, 3.125e-05)[None]
+    prof = None
+    if profile == "shipped":
+        k = np.arange(16)
+        prof = (8000 * (k / 16 - np.sin(2 * np.pi * k / 16) / (2 * np.pi)))[::-1].astype(np.float32)
+    got, want, f, wf = _run_pair(v, a, b, True, sc, src, mc, steps, pml=(8, 0, prof))
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+def test_fp64_bitexact():
+    _case("fp64 vs oracle f64")
+    v, a, b, rng = _closed_room(25)
+    sc, mc = _special_cells(v, rng)
+    steps = 150
+    dt = rooms.courant_dt(0.02)
+    src = np.stack([rooms.ricker(steps, dt)] * len(sc))
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, precision="fp64")
+    # the library returns FP32 copies of FP64 state
+    _assert_bitexact(got, want.astype(np.float32), "traces")
+    _assert_bitexact(f, wf.astype(np.float32), "field")
+
+
+def test_batched_rooms_match_single():
+    _case("batched rooms vs one handle per room")
+    ws = rooms.batch_rooms(seed=7, nrooms=3, steps=120, lo=(24, 20, 18), hi=(44, 38, 30), nmic=3)
+    # single runs
+    single = []
+    for w in ws:
+        s = rooms.make_sim(w, afx.Sim)
+        s.run(w.steps)
+        single.append(s.receivers())
+        s.close()
+    # one handle, all rooms
+    s = afx.Sim(c=ws[0].c, dx=ws[0].dx, dt=ws[0].dt, res=ws[0].dx, compat=False)
+    s.set_voxels(ws[0].vox)
+    s.set_materials(ws[0].a, ws[0].b)
+    ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
+    for rid, w in zip(ids, ws):
+        for cell, sig in zip(w.src_cells, w.src):
+            s.add_source_cell(cell, sig, room=rid)
+    for rid, w in zip(ids, ws):
+        for cell in w.mic_cells:
+            s.add_receiver_cell(cell, room=rid)
+    s.run(ws[0].steps)
+    tr = s.receivers()
+    s.close()
+    k = 0
+    for w, want in zip(ws, single):
+        _assert_bitexact(tr[k:k + len(w.mic_cells)], want, w.name)
+        k += len(w.mic_cells)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_reduced_precision_error(precision):
+    _case(f"reduced precision {precision}")
+    v, a, b, rng = _closed_room(26)
+    sc = rooms.interior_cells(v, rng, 1)
+    mc = rooms.interior_cells(v, rng, 4)
+    steps = 400
+    dt = rooms.courant_dt(0.02)
+    src = rooms.ricker(steps, dt)[None]
+    got, _, _, _ = _run_pair(v, a, b, False, sc, src, mc, steps, precision=precision)
+    ref, want64, _, _ = _run_pair(v, a, b, False, sc, src, mc, steps, precision="fp64")
+    err = np.linalg.norm(got - want64) / np.linalg.norm(want64)
+    bound = 2e-2 if precision == "fp16" else 1e-1
+    print(f"{precision} rel L2 vs fp64: {err:.3e}")
+    assert err < bound
