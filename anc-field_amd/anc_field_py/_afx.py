@@ -83,6 +83,10 @@ _SIGS = {
     "afx_add_receiver_cell": (C.c_int, [_P, C.c_int32, _i32p]),
     "afx_add_room": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _i32p, _f32p, _f32p, C.c_int32, C.c_int32, _i32p]),
     "afx_run": (C.c_int, [_P, C.c_int64]),
+    "afx_synchronize": (C.c_int, [_P]),
+    "afx_voxelize_kerkythea": (C.c_int, [C.c_char_p, C.c_float, _i32p, _i32p, C.c_int64, _f32p, _f32p, C.c_int32,
+                                         _i32p, _i32p]),
+    "afx_classify": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32, C.POINTER(C.c_int64), _i32p]),
     "afx_get_receivers": (C.c_int, [_P, _f32p, C.c_int64, C.POINTER(C.c_int64)]),
     "afx_get_pressure": (C.c_int, [_P, C.c_int32, C.c_int32, _f32p, C.c_int64]),
     "afx_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
@@ -123,6 +127,31 @@ def device_name(i):
     buf = C.create_string_buffer(256)
     _check(lib.afx_device_name(i, buf, 256))
     return buf.value.decode()
+
+
+def voxelize_kerkythea(path, res=0.02):
+    """(vox[nz, ny, nx] int32, a[nmat, ncoef], b[nmat, ncoef]) that `load` builds."""
+    dims = (C.c_int32 * 3)()
+    nc, nm = C.c_int32(0), C.c_int32(0)
+    _check(lib.afx_voxelize_kerkythea(os.fsencode(path), res, dims, None, 0, None, None, 0, C.byref(nc), C.byref(nm)))
+    nx, ny, nz = dims
+    vox = np.empty((nz, ny, nx), np.int32)
+    a = np.empty((nm.value, nc.value), np.float32)
+    b = np.empty_like(a)
+    _check(lib.afx_voxelize_kerkythea(os.fsencode(path), res, dims, vox.ctypes.data_as(_i32p), vox.size,
+                                      a.ctypes.data_as(_f32p), b.ctypes.data_as(_f32p), a.size, C.byref(nc),
+                                      C.byref(nm)))
+    return vox, a, b
+
+
+def classify_counts(vox, compat=True):
+    """Boundary cells per reference group id (26 counts)."""
+    v, vp = _i32(vox)
+    nz, ny, nx = v.shape
+    counts = (C.c_int64 * 26)()
+    bad = (C.c_int32 * 3)()
+    _check(lib.afx_classify(nx, ny, nz, vp, 1 if compat else 0, counts, bad))
+    return np.array(counts[:], np.int64)
 
 
 def default_config():
@@ -239,6 +268,9 @@ class Sim:
 
     def run(self, steps):
         _check(lib.afx_run(self._h, int(steps)))
+
+    def synchronize(self):
+        _check(lib.afx_synchronize(self._h))
 
     def receivers(self):
         n = C.c_int64(0)

@@ -215,9 +215,8 @@ int check_sim(afx_sim* s)
 
 // Classify the air cells of one room (lrs-data.hpp:117-215): cells with
 // 1 <= x <= nx-2 (likewise y, z) whose 6-neighbourhood holds solid voxels.
-int classify_room(afx_sim* s, int ridx, std::vector<Entry>& out)
+int classify_volume(const Room& r, int ridx, bool compat, std::vector<Entry>& out, int64_t* group_count)
 {
-    const Room& r = s->rooms[ridx];
     if (r.vox.empty()) return AFX_OK;
     const int64_t sx = r.nx, sxy = (int64_t)r.nx * r.ny;
     auto V = [&](int64_t x, int64_t y, int64_t z) -> int32_t {
@@ -253,14 +252,14 @@ int classify_room(afx_sim* s, int ridx, std::vector<Entry>& out)
                 e.cell = r.base + z * sxy + y * sx + x;
                 e.info = m | ((uint32_t)g << kInfoGroupShift) | ((uint32_t)K << kInfoKShift) |
                          ((uint32_t)ridx << kInfoRoomShift);
-                if (s->compat && g == 25) e.info |= kInfoNoUpdate;
+                if (compat && g == 25) e.info |= kInfoNoUpdate;
                 e.mat = mw;
                 e.room = ridx;
                 e.x = (int32_t)x;
                 e.y = (int32_t)y;
                 e.z = (int32_t)z;
                 out.push_back(e);
-                s->group_count[g]++;
+                if (group_count) group_count[g]++;
             }
     return AFX_OK;
 }
@@ -643,7 +642,7 @@ int commit(afx_sim* s)
     s->entries.clear();
     if (s->has_model)
         for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
-            int rc = classify_room(s, (int)ri, s->entries);
+            int rc = classify_volume(s->rooms[ri], (int)ri, s->compat, s->entries, s->group_count);
             if (rc) return rc;
         }
     std::sort(s->entries.begin(), s->entries.end(), [](const Entry& a, const Entry& b) { return a.cell < b.cell; });
@@ -687,10 +686,10 @@ int commit(afx_sim* s)
         HIP_TRY(upload(s->d_B, b, s->stream));
     }
     // fields
-    const size_t fbytes = prec_bytes(s->prec) * (size_t)s->total_cells;
+    const size_t fbytes = (prec_bytes(s->prec) * (size_t)s->total_cells + 3) / 4 * 4;
     for (auto& p : s->P) {
         HIP_TRY(p.alloc(fbytes));
-        HIP_TRY(launch_fill(p.p, (int64_t)((fbytes + 3) / 4 * 4), s->stream));
+        HIP_TRY(launch_fill(p.p, (int64_t)fbytes, s->stream));
     }
     s->rot[0] = 0; s->rot[1] = 1; s->rot[2] = 2;
     HIP_TRY(s->d_t.alloc(sizeof(int64_t)));
@@ -1305,8 +1304,8 @@ int afx_get_receivers(afx_sim* s, float* out, int64_t cap, int64_t* n_written)
     const int64_t nm = (int64_t)s->mics.size();
     const int64_t T = s->t;
     if (n_written) *n_written = nm * T;
-    if (nm == 0 || T == 0) return AFX_OK;
-    if (!out || cap < nm * T) return fail(AFX_E_ARG, "buffer holds %lld values, %lld needed", (long long)cap, (long long)(nm * T));
+    if (nm == 0 || T == 0 || !out) return AFX_OK;  // NULL out: size query
+    if (cap < nm * T) return fail(AFX_E_ARG, "buffer holds %lld values, %lld needed", (long long)cap, (long long)(nm * T));
     HIP_TRY(hipSetDevice(s->cfg.device));
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy2D(out, sizeof(float) * (size_t)T, s->d_traces.p, sizeof(float) * (size_t)s->trace_cap,
@@ -1330,6 +1329,58 @@ int afx_get_pressure(afx_sim* s, int32_t room, int32_t which, float* out, int64_
     HIP_TRY(launch_to_f32(src, s->prec, tmp.as<float>(), n, s->scale, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy(out, tmp.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return AFX_OK;
+}
+
+int afx_voxelize_kerkythea(const char* path, float res, int32_t dims[3], int32_t* vox, int64_t vox_cap, float* a,
+                           float* b, int32_t coef_cap, int32_t* ncoef, int32_t* nmat)
+{
+    if (!path || !dims || !ncoef || !nmat || !(res > 0)) return fail(AFX_E_ARG, "null argument or bad resolution");
+    std::vector<int32_t> v;
+    std::vector<float> va, vb;
+    std::string err;
+    int rc = afx::load_kerkythea(path, res, dims, v, va, vb, ncoef, nmat, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    if (vox) {
+        if (vox_cap < (int64_t)v.size()) return fail(AFX_E_ARG, "voxel buffer holds %lld, %zu needed", (long long)vox_cap, v.size());
+        std::copy(v.begin(), v.end(), vox);
+    }
+    if (a || b) {
+        if (!a || !b || coef_cap < (int32_t)va.size()) return fail(AFX_E_ARG, "coefficient buffers too small");
+        std::copy(va.begin(), va.end(), a);
+        std::copy(vb.begin(), vb.end(), b);
+    }
+    return AFX_OK;
+}
+
+int afx_classify(int32_t nx, int32_t ny, int32_t nz, const int32_t* vox, int32_t compat, int64_t counts[26],
+                 int32_t* bad_cell)
+{
+    if (!vox || !counts || nx < 3 || ny < 3 || nz < 3) return fail(AFX_E_ARG, "bad classify arguments");
+    Room r;
+    r.nx = r.vx = nx;
+    r.ny = r.vy = ny;
+    r.nz = r.vz = nz;
+    r.vox.assign(vox, vox + (size_t)nx * ny * nz);
+    r.nmat = kMaxMaterials;
+    std::vector<Entry> out;
+    std::fill(counts, counts + 26, 0);
+    int rc = classify_volume(r, 0, compat != 0, out, counts);
+    if (rc == AFX_E_GEOMETRY && bad_cell) {
+        int x = -1, y = -1, z = -1;
+        std::sscanf(g_err.c_str(), "Unphysical surfaces encountered at room %*d cell (%d,%d,%d)", &x, &y, &z);
+        bad_cell[0] = x;
+        bad_cell[1] = y;
+        bad_cell[2] = z;
+    }
+    return rc;
+}
+
+int afx_synchronize(afx_sim* s)
+{
+    if (int rc = check_sim(s)) return rc;
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    HIP_TRY(hipDeviceSynchronize());
     return AFX_OK;
 }
 

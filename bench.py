@@ -68,22 +68,20 @@ def main():
               f"(the multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
+    # torch is not imported: it bundles its own HIP runtime, which must not
+    # share a process with the one libafx.so links.  The timed region is
+    # bracketed by device-wide synchronisation through the library instead.
     from anc_field_py import _afx, rooms
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is optional for the measurement itself
-        sync = lambda: None  # noqa: E731
 
     total = args.warmup + args.steps + args.profile_steps
     w = rooms.furnished(steps=total)
     sim = rooms.make_sim(w, _afx.Sim, compat=args.compat, precision=args.precision)
     sim.run(args.warmup)
     tile = sim.autotune(30) if args.tune else None
-    sync()
+    sim.synchronize()
     t0 = time.perf_counter()
     sim.run(args.steps)
-    sync()
+    sim.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     gcells = w.cells * args.steps / elapsed / 1e9

@@ -20,6 +20,7 @@
  *   afx_get_stats                        (boundary counts logged at lrs-data.hpp:212-214)
  *   afx_autotune                         (no counterpart: tile-shape search)
  *   afx_profile_run                      (no counterpart: per-kernel event timing)
+ *   afx_synchronize                      clh Device::queue().finish(), simulation.hpp:407
  *
  * Conventions: every function returns AFX_OK (0) or a negative AFX_E_* code
  * and sets a thread-local message readable with afx_last_error().  No C++
@@ -135,7 +136,20 @@ int afx_add_room(afx_sim* sim, int32_t nx, int32_t ny, int32_t nz, const int32_t
 
 /* Advance every room by `steps` (resumable: a second call continues in time). */
 int afx_run(afx_sim* sim, int64_t steps);
-/* Receiver traces [nmic][T] with T = steps run so far, in receiver order. */
+/* Host-only helpers (no device needed).
+ * afx_voxelize_kerkythea: the voxel grid and material filters `load` would
+ * build (kerkythea.hpp:133-196); NULL buffers only report dims/ncoef/nmat.
+ * afx_classify: boundary cells per reference group (lrs-data.hpp:117-215);
+ * AFX_E_GEOMETRY with the offending cell in bad_cell on an unphysical pattern. */
+int afx_voxelize_kerkythea(const char* model_xml, float res, int32_t dims[3], int32_t* vox, int64_t vox_cap,
+                           float* a, float* b, int32_t coef_cap, int32_t* ncoef, int32_t* nmat);
+int afx_classify(int32_t nx, int32_t ny, int32_t nz, const int32_t* vox, int32_t compat, int64_t counts[26],
+                 int32_t* bad_cell);
+
+/* Waits until all work on the handle's device has finished (hipDeviceSynchronize). */
+int afx_synchronize(afx_sim* sim);
+/* Receiver traces [nmic][T] with T = steps run so far, in receiver order;
+ * out == NULL only reports the count in *n_written. */
 int afx_get_receivers(afx_sim* sim, float* out, int64_t cap, int64_t* n_written);
 /* which: 0 = newest pressure p^{n+1}, 1 = p^n, 2 = p^{n-1}; FP32 copy of room `room` */
 int afx_get_pressure(afx_sim* sim, int32_t room, int32_t which, float* out, int64_t cap);
