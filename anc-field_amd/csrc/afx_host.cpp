@@ -177,7 +177,7 @@ struct afx_sim {
     int64_t t = 0;  // host mirror of the step counter
     double scale = 1.0;
     // pml device state
-    DBuf d_phi[3], d_phi_old[3], d_psi, d_psi_old, d_z[3], d_zh[3], d_boxes, d_axis_off;
+    DBuf d_phi[3], d_phi_old[3], d_psi, d_psi_old, d_z[3], d_zh[3], d_boxes;
     int32_t nbox = 0;
     int64_t box_cells_max = 0;
     int pml_rot = 0;
@@ -432,6 +432,15 @@ int build_special(afx_sim* s)
             if (it != s->entries.end() && it->cell == q.cell) {
                 b = (int32_t)(it - s->entries.begin());
                 info[(size_t)b] |= kInfoSpecial;
+            } else if (s->pml_mode == 2 && q.room == 0) {
+                // a PML cell: its value (and psi) come from the PML update; the
+                // special pass only adds the sources to the stored value
+                const int w = s->pml_width;
+                const int c3[3] = {q.x, q.y, q.z}, n3[3] = {r.nx, r.ny, r.nz};
+                for (int k = 0; k < 3; ++k) {
+                    const bool lo = (s->pml_faces >> (2 * k)) & 1u, hi = (s->pml_faces >> (2 * k + 1)) & 1u;
+                    if ((lo && c3[k] < 1 + w) || (hi && c3[k] >= n3[k] - 1 - w)) b = -3;
+                }
             }
         }
         bidx.push_back(b);
@@ -589,8 +598,6 @@ int setup_pml_t(afx_sim* s)
     }
     s->nbox = (int32_t)boxes.size();
     HIP_TRY(upload(s->d_boxes, bx, s->stream));
-    std::vector<int32_t> aoff(3 * s->rooms.size(), 0);
-    HIP_TRY(upload(s->d_axis_off, aoff, s->stream));
     for (int d = 0; d < 3; ++d) {
         HIP_TRY(s->d_phi[d].alloc(cbytes));
         HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)cbytes, s->stream));
@@ -651,11 +658,21 @@ int commit(afx_sim* s)
     std::vector<RoomDev> rd;
     for (auto& r : s->rooms) {
         RoomDev d;
+        std::memset(&d, 0, sizeof d);
         d.base = r.base;
         d.nx = r.nx;
         d.ny = r.ny;
         d.nz = r.nz;
-        d.pad = 0;
+        const int dims[3] = {r.nx, r.ny, r.nz};
+        const bool pml = s->pml_mode == 2 && rd.empty();  // the PML belongs to room 0
+        d.pml = pml ? 1 : 0;
+        for (int k = 0; k < 3; ++k) {
+            const bool lo = pml && ((s->pml_faces >> (2 * k)) & 1u);
+            const bool hi = pml && ((s->pml_faces >> (2 * k + 1)) & 1u);
+            d.pml_lo[k] = lo ? 1 + s->pml_width : 0;
+            d.pml_hi[k] = hi ? dims[k] - 1 - s->pml_width : dims[k];
+            d.axis_off[k] = 0;
+        }
         rd.push_back(d);
     }
     HIP_TRY(upload(s->d_rooms, rd, s->stream));
@@ -789,6 +806,13 @@ StepArgs make_args(afx_sim* s)
     a.scale = s->scale;
     a.d_t = s->d_t.as<int64_t>();
     a.t_rel = 0;
+    for (int d = 0; d < 3; ++d) {
+        a.pml_phi[d] = s->d_phi[d].p;
+        a.pml_z[d] = s->d_z[d].p;
+    }
+    a.pml_psi = s->d_psi.p;
+    a.dt = s->cfg.dt;
+    a.dx = s->cfg.dx;
     return a;
 }
 
@@ -816,7 +840,6 @@ PmlArgs make_pml(afx_sim* s, const StepArgs& st)
     p.box_cells_max = s->box_cells_max;
     p.rooms = s->d_rooms.as<RoomDev>();
     p.nrooms = (int32_t)s->rooms.size();
-    p.axis_off = s->d_axis_off.as<int32_t>();
     p.l2 = st.l2;
     p.c = s->cfg.c;
     p.dx = s->cfg.dx;
@@ -825,7 +848,7 @@ PmlArgs make_pml(afx_sim* s, const StepArgs& st)
     return p;
 }
 
-bool use_fused(const afx_sim* s) { return s->pml_mode == 0 && !(s->cfg.flags & AFX_SPLIT) && s->fused_ok; }
+bool use_fused(const afx_sim* s) { return s->pml_mode != 1 && !(s->cfg.flags & AFX_SPLIT) && s->fused_ok; }
 
 // Enqueue `k` steps starting at the device counter, then advance it by k.
 // Buffer roles rotate like simulation.hpp:360.  local0 is the step index
@@ -879,6 +902,12 @@ int enqueue_steps(afx_sim* s, int k, int64_t local0, Prof* prof = nullptr)
             HIP_TRY(launch_fused(a, s->d_tile_eoff.as<int32_t>(), s->d_tile_eperm.as<int32_t>(),
                                  s->d_tile_spoff.as<int32_t>(), s->stream));
             PROF_END(0);
+            if (s->pml_mode == 2) {
+                PmlArgs p = make_pml(s, a);
+                PROF_BEGIN(3);
+                HIP_TRY(launch_pml_phys_phi(p, s->stream));
+                PROF_END(3);
+            }
             continue;
         }
         PROF_BEGIN(0);

@@ -11,6 +11,10 @@ namespace afx {
 struct RoomDev {
     int64_t base;
     int32_t nx, ny, nz;
+    int32_t pml;         // 1 when the room has a physical PML
+    // a cell is in the PML slabs when x < pml_lo[0] or x >= pml_hi[0] (likewise y, z)
+    int32_t pml_lo[3], pml_hi[3];
+    int32_t axis_off[3]; // offset of this room's per-axis zeta profiles
     int32_t pad;
 };
 
@@ -83,6 +87,11 @@ struct StepArgs {
     // time
     const int64_t* d_t;         // device step counter (start of the chunk)
     int32_t t_rel;              // step offset inside the chunk
+    // physical PML state used by the fused step kernel (compute type)
+    void* pml_phi[3];
+    void* pml_psi;
+    const void* pml_z[3];
+    double dt, dx;
 };
 
 // PML arguments
@@ -100,7 +109,6 @@ struct PmlArgs {
     int64_t box_cells_max;      // largest box volume (grid sizing)
     const RoomDev* rooms;
     int32_t nrooms;
-    const int32_t* axis_off;    // [nrooms][3] offset of each room's per-axis profile
     double l2, c, dx, dt;
     int32_t width;
 };

@@ -31,15 +31,57 @@ __device__ __forceinline__ void st(S* __restrict__ p, int64_t i, C v) { p[i] = s
 template <typename S, typename C>
 __device__ __forceinline__ C rt(C v) { return static_cast<C>(static_cast<S>(v)); }
 
+// ---------------------------------------------------------------- PML value
+// Physical PML state (compute type C).  phi_x lives at x+1/2 and is stored at
+// index x (likewise y, z); psi is cell-centred at half steps.
+template <typename C> struct PmlDev {
+    C* phx;
+    C* phy;
+    C* phz;
+    C* psi;
+    const C* z[3];
+    C dt, dx;
+};
+
+// Grote-Sim pressure update of one PML cell from the 6-neighbour sum `sum`
+// of p^n, with psi advanced in place.  Shared by the split and fused paths.
+template <typename C>
+__device__ __forceinline__ C pml_value(C sum, C pc, C pm, int64_t i, int64_t sy, int64_t sz, C sx, C syy, C szz,
+                                       const PmlDev<C>& q, C l2)
+{
+    const C dt = q.dt, dx = q.dx;
+    const C s1 = sx + syy + szz;
+    const C s2 = sx * syy + syy * szz + szz * sx;
+    const C s3 = sx * syy * szz;
+    const C dv = (q.phx[i] - q.phx[i - 1]) + (q.phy[i] - q.phy[i - sy]) + (q.phz[i] - q.phz[i - sz]);
+    const C ps = q.psi[i];
+    const C psn = ps + dt * pc;
+    C num = l2 * (sum - (C)6.0 * pc);
+    num = num + (dt * dt / dx) * dv;
+    num = num - (dt * dt) * s3 * (C)0.5 * (psn + ps);
+    num = num - (dt * dt) * s2 * pc;
+    num = num + (C)2.0 * pc - pm;
+    num = num + (C)0.5 * dt * s1 * pm;
+    q.psi[i] = psn;
+    return num / ((C)1.0 + (C)0.5 * dt * s1);
+}
+
+__device__ __forceinline__ bool in_pml(const RoomDev& r, int x, int y, int z)
+{
+    return x < r.pml_lo[0] || x >= r.pml_hi[0] || y < r.pml_lo[1] || y >= r.pml_hi[1] || z < r.pml_lo[2] ||
+           z >= r.pml_hi[2];
+}
+
 // ---------------------------------------------------------------- interior
 // One thread owns an (x, y) column of a TX x TY x TZ tile and marches z,
 // carrying p^n at z-1, z, z+1 in registers; x and y neighbours come from the
 // same and adjacent 64-lane rows (L1 / L2 hits).  The sum order is the
-// reference's: x+1, x-1, y+1, y-1, z+1, z-1.
-template <typename S, typename C, int TZ>
+// reference's: x+1, x-1, y+1, y-1, z+1, z-1.  With PML, cells inside the
+// room's PML slabs take the PML update instead.
+template <typename S, typename C, int TZ, bool PML>
 __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __restrict__ cur,
                                                 const S* __restrict__ prev, const RoomDev& r, int x, int y,
-                                                int z0, C l2, C c2)
+                                                int z0, C l2, C c2, const PmlDev<C>& q)
 {
     if (x < 1 || x >= r.nx - 1 || y < 1 || y >= r.ny - 1) return;
     const int zs = z0 < 1 ? 1 : z0;
@@ -49,6 +91,13 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
     int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + x;
     C pd = ld<S, C>(cur, i - sz);
     C pc = ld<S, C>(cur, i);
+    C sx = 0, syy = 0;
+    bool col_pml = false;
+    if (PML) {
+        sx = q.z[0][r.axis_off[0] + x];
+        syy = q.z[1][r.axis_off[1] + y];
+        col_pml = x < r.pml_lo[0] || x >= r.pml_hi[0] || y < r.pml_lo[1] || y >= r.pml_hi[1];
+    }
 #pragma unroll 4
     for (int z = zs; z < ze; ++z, i += sz) {
         const C pu = ld<S, C>(cur, i + sz);
@@ -57,8 +106,13 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
         s = s + ld<S, C>(cur, i - sy);
         s = s + pu;
         s = s + pd;
-        const C v = l2 * s + c2 * pc;
-        st<S, C>(out, i, v - ld<S, C>(prev, i));
+        const C pm = ld<S, C>(prev, i);
+        if (PML && (col_pml || z < r.pml_lo[2] || z >= r.pml_hi[2])) {
+            st<S, C>(out, i, pml_value<C>(s, pc, pm, i, sy, sz, sx, syy, q.z[2][r.axis_off[2] + z], q, l2));
+        } else {
+            const C v = l2 * s + c2 * pc;
+            st<S, C>(out, i, v - pm);
+        }
         pd = pc;
         pc = pu;
     }
@@ -76,7 +130,8 @@ __global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const
     const RoomDev r = rooms[t.x];
     const int x = t.y + (int)(threadIdx.x % TX);
     const int y = t.z + (int)(threadIdx.x / TX);
-    interior_column<S, C, TZ>(out, cur, prev, r, x, y, t.w, l2, c2);
+    const PmlDev<C> none{};
+    interior_column<S, C, TZ, false>(out, cur, prev, r, x, y, t.w, l2, c2, none);
 }
 
 // ---------------------------------------------------------------- boundary
@@ -225,6 +280,7 @@ struct BndCtx {
     float* traces;
     int64_t trace_cap;
     C scale, inv_scale;
+    PmlDev<C> pml;
 };
 
 template <typename S, typename C>
@@ -310,6 +366,13 @@ __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
     c.trace_cap = a.trace_cap;
     c.scale = (C)a.scale;
     c.inv_scale = (C)(1.0 / a.scale);
+    c.pml.phx = (C*)a.pml_phi[0];
+    c.pml.phy = (C*)a.pml_phi[1];
+    c.pml.phz = (C*)a.pml_phi[2];
+    c.pml.psi = (C*)a.pml_psi;
+    for (int d = 0; d < 3; ++d) c.pml.z[d] = (const C*)a.pml_z[d];
+    c.pml.dt = (C)a.dt;
+    c.pml.dx = (C)a.dx;
     return c;
 }
 
@@ -350,7 +413,10 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
     const RoomDev r = c.rooms[t4.x];
     const int x = t4.y + (int)(threadIdx.x % TX);
     const int y = t4.z + (int)(threadIdx.x / TX);
-    interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2);
+    if (r.pml)
+        interior_column<S, C, TZ, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml);
+    else
+        interior_column<S, C, TZ, false>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml);
     const int eb = tile_e_off[blockIdx.x], ee = tile_e_off[blockIdx.x + 1];
     const int sb = tile_sp_off[blockIdx.x], se = tile_sp_off[blockIdx.x + 1];
     if (eb == ee && sb == se) return;  // uniform per workgroup
@@ -383,23 +449,22 @@ __global__ void k_pml_p(PmlArgs a)
     S* out = (S*)a.out;
     const S* cur = (const S*)a.cur;
     const S* prev = (const S*)a.prev;
-    C* phx = (C*)a.phi[0];
-    C* phy = (C*)a.phi[1];
-    C* phz = (C*)a.phi[2];
-    C* psi = (C*)a.psi;
-    const C* zx = (const C*)a.z[0] + a.axis_off[3 * room + 0];
-    const C* zy = (const C*)a.z[1] + a.axis_off[3 * room + 1];
-    const C* zz = (const C*)a.z[2] + a.axis_off[3 * room + 2];
-    const C l2 = (C)a.l2, dx = (C)a.dx, dt = (C)a.dt;
+    PmlDev<C> pq;
+    pq.phx = (C*)a.phi[0];
+    pq.phy = (C*)a.phi[1];
+    pq.phz = (C*)a.phi[2];
+    pq.psi = (C*)a.psi;
+    pq.dt = (C)a.dt;
+    pq.dx = (C)a.dx;
+    const C* zx = (const C*)a.z[0] + r.axis_off[0];
+    const C* zy = (const C*)a.z[1] + r.axis_off[1];
+    const C* zz = (const C*)a.z[2] + r.axis_off[2];
+    const C l2 = (C)a.l2;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += (int64_t)gridDim.x * blockDim.x) {
         const int x = bx[1] + (int)(q % wx);
         const int y = bx[3] + (int)((q / wx) % wy);
         const int z = bx[5] + (int)(q / ((int64_t)wx * wy));
         const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
-        const C sx = zx[x], syy = zy[y], szz = zz[z];
-        const C s1 = sx + syy + szz;
-        const C s2 = sx * syy + syy * szz + szz * sx;
-        const C s3 = sx * syy * szz;
         const C pc = ld<S, C>(cur, i);
         const C pm = ld<S, C>(prev, i);
         C sum = ld<S, C>(cur, i + 1) + ld<S, C>(cur, i - 1);
@@ -407,16 +472,7 @@ __global__ void k_pml_p(PmlArgs a)
         sum = sum + ld<S, C>(cur, i - sy);
         sum = sum + ld<S, C>(cur, i + sz);
         sum = sum + ld<S, C>(cur, i - sz);
-        const C dv = (phx[i] - phx[i - 1]) + (phy[i] - phy[i - sy]) + (phz[i] - phz[i - sz]);
-        const C psn = psi[i] + dt * pc;
-        C num = l2 * (sum - (C)6.0 * pc);
-        num = num + (dt * dt / dx) * dv;
-        num = num - (dt * dt) * s3 * (C)0.5 * (psn + psi[i]);
-        num = num - (dt * dt) * s2 * pc;
-        num = num + (C)2.0 * pc - pm;
-        num = num + (C)0.5 * dt * s1 * pm;
-        st<S, C>(out, i, num / ((C)1.0 + (C)0.5 * dt * s1));
-        psi[i] = psn;
+        st<S, C>(out, i, pml_value<C>(sum, pc, pm, i, sy, sz, zx[x], zy[y], zz[z], pq, l2));
     }
 }
 
@@ -437,12 +493,12 @@ __global__ void k_pml_phi(PmlArgs a)
     C* phy = (C*)a.phi[1];
     C* phz = (C*)a.phi[2];
     const C* psi = (const C*)a.psi;
-    const C* zx = (const C*)a.z[0] + a.axis_off[3 * room + 0];
-    const C* zy = (const C*)a.z[1] + a.axis_off[3 * room + 1];
-    const C* zz = (const C*)a.z[2] + a.axis_off[3 * room + 2];
-    const C* hx = (const C*)a.zh[0] + a.axis_off[3 * room + 0];
-    const C* hy = (const C*)a.zh[1] + a.axis_off[3 * room + 1];
-    const C* hz = (const C*)a.zh[2] + a.axis_off[3 * room + 2];
+    const C* zx = (const C*)a.z[0] + r.axis_off[0];
+    const C* zy = (const C*)a.z[1] + r.axis_off[1];
+    const C* zz = (const C*)a.z[2] + r.axis_off[2];
+    const C* hx = (const C*)a.zh[0] + r.axis_off[0];
+    const C* hy = (const C*)a.zh[1] + r.axis_off[1];
+    const C* hz = (const C*)a.zh[2] + r.axis_off[2];
     const C c = (C)a.c, dx = (C)a.dx, dt = (C)a.dt;
     const C cc = c * c, idt = (C)1.0 / dt, idx = (C)1.0 / dx;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += (int64_t)gridDim.x * blockDim.x) {

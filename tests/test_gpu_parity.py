@@ -134,19 +134,26 @@ def test_resumable_runs():
     _assert_bitexact(f, wf, "field")
 
 
-def test_physical_pml_bitexact():
-    _case("physical PML x+ and y- faces")
+@pytest.mark.parametrize("fused", [True, False])
+def test_physical_pml_bitexact(fused):
+    _case(f"physical PML x+ and y- faces fused={fused}")
     v, a, b, rng = _closed_room(23, dims=(48, 40, 30))
     v[:, :, -1] = -1   # open x+ face
     v[:, 0, :] = -1    # open y- face
     v = rooms.repair(v)
-    sc = rooms.interior_cells(v, rng, 1)
-    mc = rooms.interior_cells(v, rng, 4)
+    inner = rooms.interior_cells(v, rng, 4)
+    # sources/receivers also inside the PML slabs and on a wall cell in a slab
+    in_pml = np.array([[44, 20, 15], [20, 3, 12]], np.int32)
+    wall_in_pml = np.array([44, 20, 1], np.int32)  # z-  wall face inside the x+ slab
+    sc = np.stack([inner[0], in_pml[0], wall_in_pml])
+    mc = np.concatenate([inner[1:], in_pml, wall_in_pml[None]])
     steps = 250
     dt = rooms.courant_dt(0.02)
-    src = rooms.ricker(steps, dt)[None]
+    src = np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 900.0, amp=0.5),
+                    rooms.ricker(steps, dt, 1300.0, amp=0.3)])
     prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
-    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, pml=(6, 2 | 4, prof))
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, pml=(6, 2 | 4, prof),
+                                 lib_kw=dict(fused=fused))
     _assert_bitexact(got, want, "traces")
     _assert_bitexact(f, wf, "field")
 
