@@ -92,6 +92,7 @@ _SIGS = {
     "afx_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "afx_autotune": (C.c_int, [_P, C.c_int32, _i32p]),
     "afx_profile_run": (C.c_int, [_P, C.c_int64, C.POINTER(Profile)]),
+    "afx_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)
@@ -305,6 +306,9 @@ class Sim:
         pr = Profile()
         _check(lib.afx_profile_run(self._h, int(steps), C.byref(pr)))
         return pr
+
+    def set_tile(self, tx, ty, tz):
+        _check(lib.afx_set_tile(self._h, tx, ty, tz))
 
     def autotune(self, steps_per_trial=30):
         t = (C.c_int32 * 3)()

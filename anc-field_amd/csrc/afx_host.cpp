@@ -162,7 +162,8 @@ struct afx_sim {
     int64_t total_cells = 0;
     DBuf P[3];
     int rot[3] = {0, 1, 2};
-    DBuf d_rooms, d_tiles, d_tile_eoff, d_tile_eperm, d_tile_spoff;
+    DBuf d_rooms, d_tiles, d_mask, d_e_xyz, d_sp_xyz, d_pml_cell, d_pml_xyz, d_pml_flag;
+    int32_t npml = 0;
     int32_t ntiles = 0;
     DBuf d_e_cell, d_e_info, d_e_mat, d_ring_in, d_ring_out, d_A, d_B;
     int32_t nb = 0;
@@ -179,7 +180,7 @@ struct afx_sim {
     // pml device state
     DBuf d_phi[3], d_phi_old[3], d_psi, d_psi_old, d_z[3], d_zh[3], d_boxes;
     int32_t nbox = 0;
-    int64_t box_cells_max = 0;
+    int32_t box_xy_max = 0, box_z_max = 0;
     int pml_rot = 0;
     // graph
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
@@ -299,57 +300,17 @@ void build_tiles(afx_sim* s, std::vector<int4>& table, std::vector<int32_t>& log
     }
 }
 
-// slot of the tile holding (room, x, y, z)
-struct TileIndex {
-    std::vector<int64_t> room_first;  // logical index of the room's first tile
-    std::vector<int32_t> slot_of_logical;
-    TileShape ts;
-    const afx_sim* s;
-    int64_t slot(int room, int x, int y, int z) const
-    {
-        const Room& r = s->rooms[room];
-        const int ntx = (r.nx + ts.tx - 1) / ts.tx, nty = (r.ny + ts.ty - 1) / ts.ty;
-        const int64_t L = room_first[room] + ((int64_t)(z / ts.tz) * nty + y / ts.ty) * ntx + x / ts.tx;
-        return slot_of_logical[(size_t)L];
-    }
-};
-
 int build_tile_tables(afx_sim* s)
 {
     std::vector<int4> table;
     std::vector<int32_t> logical_of_slot;
     build_tiles(s, table, logical_of_slot);
     s->ntiles = (int32_t)table.size();
-    TileIndex ti;
-    ti.s = s;
-    ti.ts = kTiles[s->tile];
-    int64_t acc = 0;
-    for (const Room& r : s->rooms) {
-        ti.room_first.push_back(acc);
-        acc += (int64_t)((r.nx + ti.ts.tx - 1) / ti.ts.tx) * ((r.ny + ti.ts.ty - 1) / ti.ts.ty) *
-               ((r.nz + ti.ts.tz - 1) / ti.ts.tz);
-    }
-    ti.slot_of_logical.assign((size_t)acc, -1);
-    for (size_t i = 0; i < logical_of_slot.size(); ++i)
-        if (logical_of_slot[i] >= 0) ti.slot_of_logical[(size_t)logical_of_slot[i]] = (int32_t)i;
-    // boundary entries per tile slot (indices into the linear-order entry arrays)
-    std::vector<int32_t> cnt((size_t)s->ntiles + 1, 0), eoff((size_t)s->ntiles + 1, 0), perm((size_t)s->nb);
-    std::vector<int64_t> eslot((size_t)s->nb);
-    for (int32_t e = 0; e < s->nb; ++e) {
-        const Entry& en = s->entries[(size_t)e];
-        eslot[(size_t)e] = ti.slot(en.room, en.x, en.y, en.z);
-        cnt[(size_t)eslot[(size_t)e]]++;
-    }
-    for (int32_t i = 0; i < s->ntiles; ++i) eoff[(size_t)i + 1] = eoff[(size_t)i] + cnt[(size_t)i];
-    {
-        std::vector<int32_t> fill(eoff.begin(), eoff.end() - 1);
-        for (int32_t e = 0; e < s->nb; ++e) perm[(size_t)fill[(size_t)eslot[(size_t)e]]++] = e;
-    }
     HIP_TRY(upload(s->d_tiles, table, s->stream));
-    HIP_TRY(upload(s->d_tile_eoff, eoff, s->stream));
-    HIP_TRY(upload(s->d_tile_eperm, perm, s->stream));
     return AFX_OK;
 }
+
+uint32_t pack_xyz(int x, int y, int z) { return (uint32_t)x | ((uint32_t)y << 11) | ((uint32_t)z << 22); }
 
 // Cells holding sources or receivers.  Each is finished by one thread that
 // computes the step value, adds the sources in order and records receivers.
@@ -395,33 +356,17 @@ int build_special(afx_sim* s)
     std::vector<uint32_t> info((size_t)s->nb);
     for (int32_t e = 0; e < s->nb; ++e) info[(size_t)e] = s->entries[(size_t)e].info & ~kInfoSpecial;
     std::vector<int32_t> cell, room, bidx, soff(1, 0), sids, moff(1, 0), mids;
-    // order specials by tile slot for the fused kernel
-    TileIndex ti;
-    ti.s = s;
-    ti.ts = kTiles[s->tile];
-    std::vector<int4> table;
-    std::vector<int32_t> logical_of_slot;
-    build_tiles(s, table, logical_of_slot);
-    int64_t acc = 0;
-    for (const Room& r : s->rooms) {
-        ti.room_first.push_back(acc);
-        acc += (int64_t)((r.nx + ti.ts.tx - 1) / ti.ts.tx) * ((r.ny + ti.ts.ty - 1) / ti.ts.ty) *
-               ((r.nz + ti.ts.tz - 1) / ti.ts.tz);
-    }
-    ti.slot_of_logical.assign((size_t)acc, -1);
-    for (size_t i = 0; i < logical_of_slot.size(); ++i)
-        if (logical_of_slot[i] >= 0) ti.slot_of_logical[(size_t)logical_of_slot[i]] = (int32_t)i;
-    std::vector<std::pair<int64_t, size_t>> order;
-    for (size_t k = 0; k < sp.size(); ++k) order.push_back({ti.slot(sp[k].room, sp[k].x, sp[k].y, sp[k].z), k});
-    std::sort(order.begin(), order.end());
-    std::vector<int32_t> tsoff((size_t)s->ntiles + 1, 0);
-    for (auto& o : order) tsoff[(size_t)o.first + 1]++;
-    for (int32_t i = 0; i < s->ntiles; ++i) tsoff[(size_t)i + 1] += tsoff[(size_t)i];
-    for (auto& o : order) {
-        const Sp& q = sp[o.second];
+    std::vector<uint32_t> xyz;
+    // cells the fused kernel's interior tiles leave to other threads: every
+    // boundary cell and every source/receiver cell
+    std::vector<uint32_t> mask((size_t)((s->total_cells + 31) / 32), 0u);
+    for (const Entry& e : s->entries) mask[(size_t)(e.cell >> 5)] |= 1u << (e.cell & 31);
+    for (const Sp& q : sp) {
         const Room& r = s->rooms[q.room];
+        mask[(size_t)(q.cell >> 5)] |= 1u << (q.cell & 31);
         cell.push_back((int32_t)q.cell);
         room.push_back(q.room);
+        xyz.push_back(pack_xyz(q.x, q.y, q.z));
         int32_t b = -1;
         const bool edge = q.x == 0 || q.y == 0 || q.z == 0 || q.x == r.nx - 1 || q.y == r.ny - 1 || q.z == r.nz - 1;
         if (edge) {
@@ -458,7 +403,50 @@ int build_special(afx_sim* s)
     HIP_TRY(upload(s->d_sp_mic_off, moff, s->stream));
     HIP_TRY(upload(s->d_sp_mic, mids, s->stream));
     HIP_TRY(upload(s->d_e_info, info, s->stream));
-    HIP_TRY(upload(s->d_tile_spoff, tsoff, s->stream));
+    HIP_TRY(upload(s->d_sp_xyz, xyz, s->stream));
+    // fused path: every cell of room 0's PML slabs (solid ones included) is
+    // one PML thread; boundary cells there only advance phi and psi, PML
+    // source/receiver cells are finished by their PML thread
+    std::vector<int32_t> pcell, pflag;
+    std::vector<uint32_t> pxyz;
+    if (s->pml_mode == 2) {
+        const Room& r = s->rooms[0];
+        const int w = s->pml_width;
+        int lo[3], hi[3];
+        const int n3[3] = {r.nx, r.ny, r.nz};
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = ((s->pml_faces >> (2 * k)) & 1u) ? 1 + w : 0;
+            hi[k] = ((s->pml_faces >> (2 * k + 1)) & 1u) ? n3[k] - 1 - w : n3[k];
+        }
+        std::vector<std::pair<int64_t, int32_t>> pml_sp;  // (cell, special index) of PML cells
+        for (size_t j = 0; j < bidx.size(); ++j)
+            if (bidx[j] == -3) pml_sp.push_back({cell[j], (int32_t)j});
+        std::sort(pml_sp.begin(), pml_sp.end());
+        for (int z = 1; z < r.nz - 1; ++z)
+            for (int y = 1; y < r.ny - 1; ++y)
+                for (int x = 1; x < r.nx - 1; ++x) {
+                    if (!(x < lo[0] || x >= hi[0] || y < lo[1] || y >= hi[1] || z < lo[2] || z >= hi[2])) continue;
+                    const int64_t c = r.base + ((int64_t)z * r.ny + y) * r.nx + x;
+                    int32_t flag = -1;
+                    auto it = std::lower_bound(s->entries.begin(), s->entries.end(), c,
+                                               [](const Entry& e, int64_t v) { return e.cell < v; });
+                    if (it != s->entries.end() && it->cell == c) {
+                        flag = -2;
+                    } else {
+                        auto q = std::lower_bound(pml_sp.begin(), pml_sp.end(), std::make_pair(c, (int32_t)-1));
+                        if (q != pml_sp.end() && q->first == c) flag = q->second;
+                    }
+                    pcell.push_back((int32_t)c);
+                    pxyz.push_back(pack_xyz(x, y, z));
+                    pflag.push_back(flag);
+                    mask[(size_t)(c >> 5)] |= 1u << (c & 31);
+                }
+    }
+    s->npml = (int32_t)pcell.size();
+    HIP_TRY(upload(s->d_pml_cell, pcell, s->stream));
+    HIP_TRY(upload(s->d_pml_xyz, pxyz, s->stream));
+    HIP_TRY(upload(s->d_pml_flag, pflag, s->stream));
+    HIP_TRY(upload(s->d_mask, mask, s->stream));
     // sources
     int64_t cap = 1;
     for (auto& q : s->srcs) cap = std::max<int64_t>(cap, (int64_t)q.samples.size());
@@ -590,20 +578,28 @@ int setup_pml_t(afx_sim* s)
         }
     }
     std::vector<int32_t> bx;
-    s->box_cells_max = 0;
+    s->box_xy_max = 0;
+    s->box_z_max = 0;
     for (const auto& b : boxes) {
         bx.push_back(0);
         for (int k = 0; k < 6; ++k) bx.push_back(b[(size_t)k]);
-        s->box_cells_max = std::max<int64_t>(s->box_cells_max, (int64_t)(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]));
+        s->box_xy_max = std::max(s->box_xy_max, (b[1] - b[0]) * (b[3] - b[2]));
+        s->box_z_max = std::max(s->box_z_max, b[5] - b[4]);
     }
     s->nbox = (int32_t)boxes.size();
     HIP_TRY(upload(s->d_boxes, bx, s->stream));
+    // two sets: the fused path alternates them by step parity, the split
+    // path updates set 0 in place
     for (int d = 0; d < 3; ++d) {
         HIP_TRY(s->d_phi[d].alloc(cbytes));
         HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)cbytes, s->stream));
+        HIP_TRY(s->d_phi_old[d].alloc(cbytes));
+        HIP_TRY(launch_fill(s->d_phi_old[d].p, (int64_t)cbytes, s->stream));
     }
     HIP_TRY(s->d_psi.alloc(cbytes));
     HIP_TRY(launch_fill(s->d_psi.p, (int64_t)cbytes, s->stream));
+    HIP_TRY(s->d_psi_old.alloc(cbytes));
+    HIP_TRY(launch_fill(s->d_psi_old.p, (int64_t)cbytes, s->stream));
     return AFX_OK;
 }
 
@@ -626,8 +622,12 @@ int commit(afx_sim* s)
             return fail(AFX_E_RANGE, "minsize grid %dx%dx%d is smaller than the model volume %dx%dx%d", r0.nx, r0.ny,
                         r0.nz, r0.vx, r0.vy, r0.vz);
     }
-    for (auto& r : s->rooms)
+    for (auto& r : s->rooms) {
         if (r.nx < 3 || r.ny < 3 || r.nz < 3) return fail(AFX_E_ARG, "grid %dx%dx%d too small", r.nx, r.ny, r.nz);
+        // packed cell coordinates hold 11, 11 and 10 bits
+        if (r.nx > 2047 || r.ny > 2047 || r.nz > 1023)
+            return fail(AFX_E_RANGE, "grid %dx%dx%d above the 2047x2047x1023 limit", r.nx, r.ny, r.nz);
+    }
     if (s->has_model && s->ncoef <= 0) return fail(AFX_E_STATE, "voxels set but no materials");
     if (s->ncoef > kMaxCoef) return fail(AFX_E_ARG, "ncoef %d above the supported %d", s->ncoef, kMaxCoef);
     if ((int)s->rooms.size() > kMaxRooms) return fail(AFX_E_ARG, "more than %d rooms", kMaxRooms);
@@ -677,13 +677,16 @@ int commit(afx_sim* s)
     }
     HIP_TRY(upload(s->d_rooms, rd, s->stream));
     std::vector<int32_t> ecell((size_t)s->nb);
-    std::vector<uint32_t> emat((size_t)s->nb);
+    std::vector<uint32_t> emat((size_t)s->nb), exyz((size_t)s->nb);
     for (int32_t e = 0; e < s->nb; ++e) {
-        ecell[(size_t)e] = (int32_t)s->entries[(size_t)e].cell;
-        emat[(size_t)e] = s->entries[(size_t)e].mat;
+        const Entry& en = s->entries[(size_t)e];
+        ecell[(size_t)e] = (int32_t)en.cell;
+        emat[(size_t)e] = en.mat;
+        exyz[(size_t)e] = pack_xyz(en.x, en.y, en.z);
     }
     HIP_TRY(upload(s->d_e_cell, ecell, s->stream));
     HIP_TRY(upload(s->d_e_mat, emat, s->stream));
+    HIP_TRY(upload(s->d_e_xyz, exyz, s->stream));
     const size_t cbytes = s->prec == Prec::F64 ? 8 : 4;
     const int N = std::max(s->ncoef, 1);
     const size_t rbytes = cbytes * (size_t)3 * N * (size_t)std::max(s->nb, 1);
@@ -807,12 +810,23 @@ StepArgs make_args(afx_sim* s)
     a.d_t = s->d_t.as<int64_t>();
     a.t_rel = 0;
     for (int d = 0; d < 3; ++d) {
-        a.pml_phi[d] = s->d_phi[d].p;
+        a.pml_phi[0][d] = s->d_phi[d].p;
+        a.pml_phi[1][d] = s->d_phi_old[d].p;
         a.pml_z[d] = s->d_z[d].p;
+        a.pml_zh[d] = s->d_zh[d].p;
     }
-    a.pml_psi = s->d_psi.p;
+    a.pml_psi[0] = s->d_psi.p;
+    a.pml_psi[1] = s->d_psi_old.p;
+    a.pml_cell = s->d_pml_cell.as<int32_t>();
+    a.pml_xyz = s->d_pml_xyz.as<uint32_t>();
+    a.pml_flag = s->d_pml_flag.as<int32_t>();
+    a.npml = s->pml_mode == 2 ? s->npml : 0;
+    a.mask = s->d_mask.as<uint32_t>();
+    a.e_xyz = s->d_e_xyz.as<uint32_t>();
+    a.sp_xyz = s->d_sp_xyz.as<uint32_t>();
     a.dt = s->cfg.dt;
     a.dx = s->cfg.dx;
+    a.c = s->cfg.c;
     return a;
 }
 
@@ -837,7 +851,8 @@ PmlArgs make_pml(afx_sim* s, const StepArgs& st)
     }
     p.boxes = s->d_boxes.as<int32_t>();
     p.nbox = s->nbox;
-    p.box_cells_max = s->box_cells_max;
+    p.box_xy_max = s->box_xy_max;
+    p.box_z_max = s->box_z_max;
     p.rooms = s->d_rooms.as<RoomDev>();
     p.nrooms = (int32_t)s->rooms.size();
     p.l2 = st.l2;
@@ -899,15 +914,8 @@ int enqueue_steps(afx_sim* s, int k, int64_t local0, Prof* prof = nullptr)
         a.t_rel = j;
         if (use_fused(s)) {
             PROF_BEGIN(0);
-            HIP_TRY(launch_fused(a, s->d_tile_eoff.as<int32_t>(), s->d_tile_eperm.as<int32_t>(),
-                                 s->d_tile_spoff.as<int32_t>(), s->stream));
+            HIP_TRY(launch_fused(a, s->stream));
             PROF_END(0);
-            if (s->pml_mode == 2) {
-                PmlArgs p = make_pml(s, a);
-                PROF_BEGIN(3);
-                HIP_TRY(launch_pml_phys_phi(p, s->stream));
-                PROF_END(3);
-            }
             continue;
         }
         PROF_BEGIN(0);
@@ -1547,6 +1555,22 @@ struct TuneKey {
 std::map<TuneKey, int> g_tune_cache;
 
 }  // namespace
+
+extern "C" int afx_set_tile(afx_sim* s, int32_t tx, int32_t ty, int32_t tz)
+{
+    if (int rc = check_sim(s)) return rc;
+    for (int k = 0; k < kNumTiles; ++k)
+        if (kTiles[k].tx == tx && kTiles[k].ty == ty && kTiles[k].tz == tz) {
+            if (!s->committed) {
+                s->tile = k;
+                return AFX_OK;
+            }
+            HIP_TRY(hipSetDevice(s->cfg.device));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            return set_tile(s, k);
+        }
+    return fail(AFX_E_ARG, "tile %dx%dx%d is not one of the compiled shapes", tx, ty, tz);
+}
 
 extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3])
 {

@@ -87,11 +87,23 @@ struct StepArgs {
     // time
     const int64_t* d_t;         // device step counter (start of the chunk)
     int32_t t_rel;              // step offset inside the chunk
-    // physical PML state used by the fused step kernel (compute type)
-    void* pml_phi[3];
-    void* pml_psi;
+    // fused step kernel: cells finished outside the interior tiles, and the
+    // packed coordinates (x | y << 11 | z << 22) of entries and special cells
+    const uint32_t* mask;
+    const uint32_t* e_xyz;
+    const uint32_t* sp_xyz;
+    // physical PML state (compute type): phi and psi double buffers (the
+    // split path uses set 0 in place), zeta at cell centres and +1/2 points,
+    // and the fused path's PML cell list
+    void* pml_phi[2][3];
+    void* pml_psi[2];
     const void* pml_z[3];
-    double dt, dx;
+    const void* pml_zh[3];
+    const int32_t* pml_cell;
+    const uint32_t* pml_xyz;
+    const int32_t* pml_flag;
+    int32_t npml;
+    double dt, dx, c;
 };
 
 // PML arguments
@@ -106,7 +118,8 @@ struct PmlArgs {
     const void* zh[3];          // zeta at +1/2 points
     const int32_t* boxes;       // [nbox][7]: room, x0, x1, y0, y1, z0, z1
     int32_t nbox;
-    int64_t box_cells_max;      // largest box volume (grid sizing)
+    int32_t box_xy_max;         // largest box x*y extent (grid sizing)
+    int32_t box_z_max;          // largest box z extent
     const RoomDev* rooms;
     int32_t nrooms;
     double l2, c, dx, dt;
@@ -116,8 +129,7 @@ struct PmlArgs {
 // launchers (afx_kernels.hip)
 hipError_t launch_interior(const StepArgs& a, hipStream_t s);
 hipError_t launch_boundary(const StepArgs& a, int skip_special, int special_mode, hipStream_t s);
-hipError_t launch_fused(const StepArgs& a, const int32_t* tile_e_off, const int32_t* tile_e_perm,
-                        const int32_t* tile_sp_off, hipStream_t s);
+hipError_t launch_fused(const StepArgs& a, hipStream_t s);
 hipError_t launch_advance_t(int64_t* d_t, int64_t k, hipStream_t s);
 hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s);
 hipError_t launch_pml_phys_phi(const PmlArgs& a, hipStream_t s);

@@ -43,27 +43,47 @@ template <typename C> struct PmlDev {
     C dt, dx;
 };
 
-// Grote-Sim pressure update of one PML cell from the 6-neighbour sum `sum`
-// of p^n, with psi advanced in place.  Shared by the split and fused paths.
+// Grote-Sim pressure update of one PML cell: `sum` is the 6-neighbour sum of
+// p^n, pc/pm the cell's p^n/p^{n-1}, dv the divergence of phi^n times dx,
+// ps = psi^{n-1/2} and psn = psi^{n+1/2} = ps + dt*pc.
 template <typename C>
-__device__ __forceinline__ C pml_value(C sum, C pc, C pm, int64_t i, int64_t sy, int64_t sz, C sx, C syy, C szz,
-                                       const PmlDev<C>& q, C l2)
+__device__ __forceinline__ C pml_formula(C sum, C pc, C pm, C dv, C ps, C psn, C sx, C syy, C szz, C dt, C dx, C l2)
 {
-    const C dt = q.dt, dx = q.dx;
     const C s1 = sx + syy + szz;
     const C s2 = sx * syy + syy * szz + szz * sx;
     const C s3 = sx * syy * szz;
-    const C dv = (q.phx[i] - q.phx[i - 1]) + (q.phy[i] - q.phy[i - sy]) + (q.phz[i] - q.phz[i - sz]);
-    const C ps = q.psi[i];
-    const C psn = ps + dt * pc;
     C num = l2 * (sum - (C)6.0 * pc);
     num = num + (dt * dt / dx) * dv;
     num = num - (dt * dt) * s3 * (C)0.5 * (psn + ps);
     num = num - (dt * dt) * s2 * pc;
     num = num + (C)2.0 * pc - pm;
     num = num + (C)0.5 * dt * s1 * pm;
-    q.psi[i] = psn;
     return num / ((C)1.0 + (C)0.5 * dt * s1);
+}
+
+// Split path: phi^n and psi read from single buffers, psi advanced in place.
+template <typename C>
+__device__ __forceinline__ C pml_value(C sum, C pc, C pm, int64_t i, int64_t sy, int64_t sz, C sx, C syy, C szz,
+                                       const PmlDev<C>& q, C l2)
+{
+    const C dv = (q.phx[i] - q.phx[i - 1]) + (q.phy[i] - q.phy[i - sy]) + (q.phz[i] - q.phz[i - sz]);
+    const C ps = q.psi[i];
+    const C psn = ps + q.dt * pc;
+    q.psi[i] = psn;
+    return pml_formula<C>(sum, pc, pm, dv, ps, psn, sx, syy, szz, q.dt, q.dx, l2);
+}
+
+// Crank-Nicolson step of one phi component at a +1/2 point: `old` is phi^n,
+// (p1j, p1n) p^{n+1} at the cell and its + neighbour, (pj, pn) p^n there,
+// (psj, psn) psi^{n+1/2} there; h zeta at the half point, g2 and g3 the
+// cross-axis factors.
+template <typename C>
+__device__ __forceinline__ C phi_update(C old, C p1j, C p1n, C pj, C pn, C psj, C psn, C h, C g2, C g3, C cc,
+                                        C idt, C idx)
+{
+    const C dn = (p1n - p1j) * idx, dold = (pn - pj) * idx;
+    const C dps = (psn - psj) * idx;
+    return ((idt - (C)0.5 * h) * old + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
 }
 
 __device__ __forceinline__ bool in_pml(const RoomDev& r, int x, int y, int z)
@@ -77,11 +97,14 @@ __device__ __forceinline__ bool in_pml(const RoomDev& r, int x, int y, int z)
 // carrying p^n at z-1, z, z+1 in registers; x and y neighbours come from the
 // same and adjacent 64-lane rows (L1 / L2 hits).  The sum order is the
 // reference's: x+1, x-1, y+1, y-1, z+1, z-1.  With PML, cells inside the
-// room's PML slabs take the PML update instead.
-template <typename S, typename C, int TZ, bool PML>
+// room's PML slabs take the PML update instead.  With MASK, cells whose bit
+// is set in `mask` (boundary and source/receiver cells, finished by other
+// threads of the same launch) are left alone, PML state included.
+template <typename S, typename C, int TZ, bool PML, bool MASK>
 __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __restrict__ cur,
                                                 const S* __restrict__ prev, const RoomDev& r, int x, int y,
-                                                int z0, C l2, C c2, const PmlDev<C>& q)
+                                                int z0, C l2, C c2, const PmlDev<C>& q,
+                                                const uint32_t* __restrict__ mask)
 {
     if (x < 1 || x >= r.nx - 1 || y < 1 || y >= r.ny - 1) return;
     const int zs = z0 < 1 ? 1 : z0;
@@ -89,8 +112,6 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
     if (zs >= ze) return;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + x;
-    C pd = ld<S, C>(cur, i - sz);
-    C pc = ld<S, C>(cur, i);
     C sx = 0, syy = 0;
     bool col_pml = false;
     if (PML) {
@@ -98,23 +119,60 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
         syy = q.z[1][r.axis_off[1] + y];
         col_pml = x < r.pml_lo[0] || x >= r.pml_hi[0] || y < r.pml_lo[1] || y >= r.pml_hi[1];
     }
-#pragma unroll 4
-    for (int z = zs; z < ze; ++z, i += sz) {
-        const C pu = ld<S, C>(cur, i + sz);
-        C s = ld<S, C>(cur, i + 1) + ld<S, C>(cur, i - 1);
-        s = s + ld<S, C>(cur, i + sy);
-        s = s + ld<S, C>(cur, i - sy);
-        s = s + pu;
+    // Software pipeline with two register sets A and B used alternately: the
+    // loads of plane z+1 are issued before plane z is computed and stored,
+    // and no loaded register is copied while its load may be in flight.
+    struct Plane {
+        C u, xp, xm, yp, ym, pm;  // p^n at z+1, x+-1, y+-1; p^{n-1}
+        uint32_t mw;
+    };
+    auto load_plane = [&](Plane& P, int64_t j) {
+        P.u = ld<S, C>(cur, j + sz);
+        P.xp = ld<S, C>(cur, j + 1);
+        P.xm = ld<S, C>(cur, j - 1);
+        P.yp = ld<S, C>(cur, j + sy);
+        P.ym = ld<S, C>(cur, j - sy);
+        P.pm = ld<S, C>(prev, j);
+        P.mw = MASK ? mask[j >> 5] : 0u;
+    };
+    // (pd, pc) = p^n at z-1 and z of the plane being computed
+    auto do_plane = [&](const Plane& P, int64_t j, int z, C pd, C pc) {
+        C s = P.xp + P.xm;
+        s = s + P.yp;
+        s = s + P.ym;
+        s = s + P.u;
         s = s + pd;
-        const C pm = ld<S, C>(prev, i);
-        if (PML && (col_pml || z < r.pml_lo[2] || z >= r.pml_hi[2])) {
-            st<S, C>(out, i, pml_value<C>(s, pc, pm, i, sy, sz, sx, syy, q.z[2][r.axis_off[2] + z], q, l2));
-        } else {
-            const C v = l2 * s + c2 * pc;
-            st<S, C>(out, i, v - pm);
+        const bool skip = MASK && ((P.mw >> (j & 31)) & 1u);
+        if (!skip) {
+            if (PML && (col_pml || z < r.pml_lo[2] || z >= r.pml_hi[2])) {
+                st<S, C>(out, j, pml_value<C>(s, pc, P.pm, j, sy, sz, sx, syy, q.z[2][r.axis_off[2] + z], q, l2));
+            } else {
+                const C v = l2 * s + c2 * pc;
+                st<S, C>(out, j, v - P.pm);
+            }
         }
-        pd = pc;
-        pc = pu;
+    };
+    // Planes go in pairs (z, z1 = min(z+1, ze-1)); with an odd count the last
+    // plane is computed twice, storing the same value, and prefetches clamp
+    // to the last plane, so the loop body has no branch around a load.
+    C pd = ld<S, C>(cur, i - sz);
+    C pc = ld<S, C>(cur, i);
+    const int64_t i_last = i + (int64_t)(ze - 1 - zs) * sz;
+    Plane A, B;
+    load_plane(A, i);
+    for (int z = zs; z < ze; z += 2, i += 2 * sz) {
+        const bool odd_tail = z + 1 >= ze;  // uniform across the wave
+        const int64_t i1 = odd_tail ? i : i + sz;
+        const int64_t i2 = (z + 2 < ze) ? i + 2 * sz : i_last;
+        load_plane(B, i1);
+        do_plane(A, i, z, pd, pc);
+        const C au = A.u;  // p^n at z+1, already consumed above
+        load_plane(A, i2);
+        const C pd1 = odd_tail ? pd : pc;
+        const C pc1 = odd_tail ? pc : au;
+        do_plane(B, i1, odd_tail ? z : z + 1, pd1, pc1);
+        pd = pc1;
+        pc = B.u;
     }
 }
 
@@ -131,7 +189,7 @@ __global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const
     const int x = t.y + (int)(threadIdx.x % TX);
     const int y = t.z + (int)(threadIdx.x / TX);
     const PmlDev<C> none{};
-    interior_column<S, C, TZ, false>(out, cur, prev, r, x, y, t.w, l2, c2, none);
+    interior_column<S, C, TZ, false, false>(out, cur, prev, r, x, y, t.w, l2, c2, none, nullptr);
 }
 
 // ---------------------------------------------------------------- boundary
@@ -280,8 +338,24 @@ struct BndCtx {
     float* traces;
     int64_t trace_cap;
     C scale, inv_scale;
-    PmlDev<C> pml;
+    PmlDev<C> pml;           // split path (single phi/psi buffers)
+    // fused path PML: phi/psi double buffers selected by step parity,
+    // zeta at cell centres and +1/2 points, and the PML cell list
+    C* phib[2][3];
+    C* psib[2];
+    const C* zh[3];
+    C cc, idt, idx;
+    const int32_t* pml_cell;
+    const uint32_t* pml_xyz;
+    const int32_t* pml_flag;  // -2: boundary cell (phi/psi only), -1: plain, >= 0: special-cell index
 };
+
+__device__ __forceinline__ void unpack_xyz(uint32_t p, int& x, int& y, int& z)
+{
+    x = (int)(p & 2047u);
+    y = (int)((p >> 11) & 2047u);
+    z = (int)(p >> 22);
+}
 
 template <typename S, typename C>
 __device__ __forceinline__ void boundary_entry(const BndCtx<S, C>& c, int e, unsigned ixb, bool skip_special)
@@ -296,26 +370,11 @@ __device__ __forceinline__ void boundary_entry(const BndCtx<S, C>& c, int e, uns
     st<S, C>(c.out, i, v);
 }
 
-// mode 1: full update (boundary or leapfrog value) + sources + receivers
-// mode 2: add sources to the stored value + receivers (after a PML pass)
+// Adds special cell j's sources (in order) to its new value v, stores it and
+// records its receivers (source.cl:32-40 then mic.cl:32-41).
 template <typename S, typename C>
-__device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsigned ixb, int64_t t, int mode)
+__device__ __forceinline__ void finish_special(const BndCtx<S, C>& c, int j, int64_t i, int64_t t, C v)
 {
-    const int64_t i = c.sp_cell[j];
-    const int b = c.sp_bidx[j];
-    C v;
-    if (mode == 1 && b >= 0) {
-        const uint32_t info = c.e_info[b];
-        const RoomDev r = c.rooms[info >> kInfoRoomShift];
-        v = bnd_value<S, C>(b, i, info, c.e_mat[b], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
-                            c.compat, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k);
-    } else if (mode == 1 && b == -1) {
-        const RoomDev r = c.rooms[c.sp_room[j]];
-        v = rt<S, C>(leapfrog_value<S, C>(c.cur, c.prev, i, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k.l2, c.k.c2));
-    } else {
-        // edge planes (b == -2) and add-only mode keep the stored value
-        v = ld<S, C>(c.out, i);
-    }
     for (int q = c.sp_src_off[j]; q < c.sp_src_off[j + 1]; ++q) {
         const int s = c.sp_src[q];
         const C sv = t < c.src_len[s] ? (C)c.src[(int64_t)s * c.src_cap + t] : (C)0;
@@ -327,6 +386,113 @@ __device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsi
         const int m = c.sp_mic[q];
         c.traces[(int64_t)m * c.trace_cap + t] = (float)(stored * c.scale);
     }
+}
+
+// mode 1: full update (boundary or leapfrog value) + sources + receivers
+// mode 2: add sources to the stored value + receivers (after a PML pass)
+// mode 3: as mode 1 inside the fused launch; PML cells (b == -3) are left to
+//         the PML threads, which finish them
+template <typename S, typename C>
+__device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsigned ixb, int64_t t, int mode)
+{
+    const int64_t i = c.sp_cell[j];
+    const int b = c.sp_bidx[j];
+    const bool full = mode == 1 || mode == 3;
+    if (mode == 3 && b == -3) return;
+    C v;
+    if (full && b >= 0) {
+        const uint32_t info = c.e_info[b];
+        const RoomDev r = c.rooms[info >> kInfoRoomShift];
+        v = bnd_value<S, C>(b, i, info, c.e_mat[b], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+                            c.compat, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k);
+    } else if (full && b == -1) {
+        const RoomDev r = c.rooms[c.sp_room[j]];
+        v = rt<S, C>(leapfrog_value<S, C>(c.cur, c.prev, i, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k.l2, c.k.c2));
+    } else {
+        // edge planes (b == -2) and add-only mode keep the stored value
+        v = ld<S, C>(c.out, i);
+    }
+    finish_special<S, C>(c, j, i, t, v);
+}
+
+// ---------------------------------------------------------------- fused PML
+// Inside the fused launch of step n the auxiliary field phi^n is produced
+// from phi^{n-1}, p^n (cur), p^{n-1} (prev) and psi^{n-1/2}: the update the
+// split path runs after step n-1, moved into the next launch.  Each PML cell
+// stores its own phi^n and recomputes phi^n at its -x, -y, -z neighbours for
+// the divergence, so no cell waits on another.
+__device__ __forceinline__ bool in_region(const RoomDev& r, int x, int y, int z)
+{
+    return x >= 1 && x <= r.nx - 2 && y >= 1 && y <= r.ny - 2 && z >= 1 && z <= r.nz - 2 && in_pml(r, x, y, z);
+}
+
+template <typename S, typename C>
+__device__ __forceinline__ C phi_at(const BndCtx<S, C>& c, const RoomDev& r, int d, int64_t j, int x, int y, int z,
+                                    const C* __restrict__ phr, const C* __restrict__ psr, int64_t st)
+{
+    if (!in_region(r, x, y, z)) return (C)0;
+    const C* zc0 = c.pml.z[0] + r.axis_off[0];
+    const C* zc1 = c.pml.z[1] + r.axis_off[1];
+    const C* zc2 = c.pml.z[2] + r.axis_off[2];
+    C h, g2, g3;
+    if (d == 0) {
+        h = c.zh[0][r.axis_off[0] + x];
+        g2 = zc1[y] + zc2[z] - h;
+        g3 = zc1[y] * zc2[z];
+    } else if (d == 1) {
+        h = c.zh[1][r.axis_off[1] + y];
+        g2 = zc0[x] + zc2[z] - h;
+        g3 = zc0[x] * zc2[z];
+    } else {
+        h = c.zh[2][r.axis_off[2] + z];
+        g2 = zc0[x] + zc1[y] - h;
+        g3 = zc0[x] * zc1[y];
+    }
+    return phi_update<C>(phr[j], ld<S, C>(c.cur, j), ld<S, C>(c.cur, j + st), ld<S, C>(c.prev, j),
+                         ld<S, C>(c.prev, j + st), psr[j], psr[j + st], h, g2, g3, c.cc, c.idt, c.idx);
+}
+
+template <typename S, typename C>
+__device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int k, int64_t t)
+{
+    const int par = (int)(t & 1);
+    const C* phr0 = c.phib[par ^ 1][0];
+    const C* phr1 = c.phib[par ^ 1][1];
+    const C* phr2 = c.phib[par ^ 1][2];
+    const C* psr = c.psib[par ^ 1];
+    const RoomDev r = c.rooms[0];
+    const int64_t i = c.pml_cell[k];
+    int x, y, z;
+    unpack_xyz(c.pml_xyz[k], x, y, z);
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    const C fx = phi_at<S, C>(c, r, 0, i, x, y, z, phr0, psr, 1);
+    const C fy = phi_at<S, C>(c, r, 1, i, x, y, z, phr1, psr, sy);
+    const C fz = phi_at<S, C>(c, r, 2, i, x, y, z, phr2, psr, sz);
+    c.phib[par][0][i] = fx;
+    c.phib[par][1][i] = fy;
+    c.phib[par][2][i] = fz;
+    const C pc = ld<S, C>(c.cur, i);
+    const C ps = psr[i];
+    const C psn = ps + c.pml.dt * pc;
+    c.psib[par][i] = psn;
+    const int flag = c.pml_flag[k];
+    if (flag == -2) return;  // boundary cell: its pressure comes from the LRS update
+    const C fxm = phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, phr0, psr, 1);
+    const C fym = phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, phr1, psr, sy);
+    const C fzm = phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, phr2, psr, sz);
+    const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
+    C s = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
+    s = s + ld<S, C>(c.cur, i + sy);
+    s = s + ld<S, C>(c.cur, i - sy);
+    s = s + ld<S, C>(c.cur, i + sz);
+    s = s + ld<S, C>(c.cur, i - sz);
+    const C v = pml_formula<C>(s, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
+                               c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt, c.pml.dx,
+                               c.k.l2);
+    if (flag >= 0)
+        finish_special<S, C>(c, flag, i, t, rt<S, C>(v));
+    else
+        st<S, C>(c.out, i, v);
 }
 
 template <typename S, typename C>
@@ -366,13 +532,27 @@ __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
     c.trace_cap = a.trace_cap;
     c.scale = (C)a.scale;
     c.inv_scale = (C)(1.0 / a.scale);
-    c.pml.phx = (C*)a.pml_phi[0];
-    c.pml.phy = (C*)a.pml_phi[1];
-    c.pml.phz = (C*)a.pml_phi[2];
-    c.pml.psi = (C*)a.pml_psi;
-    for (int d = 0; d < 3; ++d) c.pml.z[d] = (const C*)a.pml_z[d];
+    c.pml.phx = (C*)a.pml_phi[0][0];
+    c.pml.phy = (C*)a.pml_phi[0][1];
+    c.pml.phz = (C*)a.pml_phi[0][2];
+    c.pml.psi = (C*)a.pml_psi[0];
+    for (int d = 0; d < 3; ++d) {
+        c.pml.z[d] = (const C*)a.pml_z[d];
+        c.zh[d] = (const C*)a.pml_zh[d];
+        c.phib[0][d] = (C*)a.pml_phi[0][d];
+        c.phib[1][d] = (C*)a.pml_phi[1][d];
+    }
+    c.psib[0] = (C*)a.pml_psi[0];
+    c.psib[1] = (C*)a.pml_psi[1];
     c.pml.dt = (C)a.dt;
     c.pml.dx = (C)a.dx;
+    const C cv = (C)a.c;
+    c.cc = cv * cv;
+    c.idt = (C)1.0 / c.pml.dt;
+    c.idx = (C)1.0 / c.pml.dx;
+    c.pml_cell = a.pml_cell;
+    c.pml_xyz = a.pml_xyz;
+    c.pml_flag = a.pml_flag;
     return c;
 }
 
@@ -395,36 +575,46 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary(BndCtx<S, C> c, int nb
 }
 
 // ---------------------------------------------------------------- fused step
-// One launch per step: each workgroup updates its tile's interior, waits on
-// a workgroup barrier, then overwrites the tile's boundary cells and
-// finishes its source/receiver cells.  Entries are sorted by tile so
-// tile_e_off/tile_sp_off (indexed like the tile table) delimit them.
+// One launch per step.  Workgroups of four kinds never touch the same cell
+// (interior tiles skip the cells flagged in `mask`):
+//   [0, nblk_b)            boundary entries, one per thread
+//   [.., +nblk_s)          source/receiver cells
+//   [.., +nblk_p)          PML cells (phi, psi and pressure)
+//   [tile0, tile0+ntiles)  interior tiles; tile0 is a multiple of 8 so the
+//                          XCD-swizzled tile order keeps its XCD mapping
+// The latency-bound lists come first so they overlap the interior stream.
 template <typename S, int TX, int TY, int TZ>
 __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>::T> c,
-                                                   const int4* __restrict__ tiles,
-                                                   const int32_t* __restrict__ tile_e_off,
-                                                   const int32_t* __restrict__ tile_e_perm,
-                                                   const int32_t* __restrict__ tile_sp_off,
+                                                   const int4* __restrict__ tiles, int ntiles, int nb, int nblk_b,
+                                                   int nsp, int nblk_s, int npml, int nblk_p, int tile0,
+                                                   const uint32_t* __restrict__ mask,
                                                    const int64_t* __restrict__ d_t, int t_rel)
 {
     using C = typename Compute<S>::T;
-    const int4 t4 = tiles[blockIdx.x];
-    if (t4.x < 0) return;  // uniform per workgroup
-    const RoomDev r = c.rooms[t4.x];
-    const int x = t4.y + (int)(threadIdx.x % TX);
-    const int y = t4.z + (int)(threadIdx.x / TX);
-    if (r.pml)
-        interior_column<S, C, TZ, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml);
-    else
-        interior_column<S, C, TZ, false>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml);
-    const int eb = tile_e_off[blockIdx.x], ee = tile_e_off[blockIdx.x + 1];
-    const int sb = tile_sp_off[blockIdx.x], se = tile_sp_off[blockIdx.x + 1];
-    if (eb == ee && sb == se) return;  // uniform per workgroup
-    __syncthreads();
+    constexpr int NT = TX * TY;
+    const int bid = (int)blockIdx.x;
+    if (bid >= tile0) {
+        const int4 t4 = tiles[bid - tile0];
+        if (t4.x < 0) return;
+        const RoomDev r = c.rooms[t4.x];
+        const int x = t4.y + (int)(threadIdx.x % TX);
+        const int y = t4.z + (int)(threadIdx.x / TX);
+        interior_column<S, C, TZ, false, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml, mask);
+        return;
+    }
     const int64_t t = *d_t + t_rel;
-    const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
-    for (int k = eb + (int)threadIdx.x; k < ee; k += TX * TY) boundary_entry<S, C>(c, tile_e_perm[k], ixb, true);
-    for (int j = sb + (int)threadIdx.x; j < se; j += TX * TY) special_entry<S, C>(c, j, ixb, t, 1);
+    if (bid < nblk_b) {
+        const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
+        const int e = bid * NT + (int)threadIdx.x;
+        if (e < nb) boundary_entry<S, C>(c, e, ixb, true);
+    } else if (bid < nblk_b + nblk_s) {
+        const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
+        const int j = (bid - nblk_b) * NT + (int)threadIdx.x;
+        if (j < nsp) special_entry<S, C>(c, j, ixb, t, 3);
+    } else if (bid < nblk_b + nblk_s + nblk_p) {
+        const int k = (bid - nblk_b - nblk_s) * NT + (int)threadIdx.x;
+        if (k < npml) pml_entry<S, C>(c, k, t);
+    }
 }
 
 __global__ void k_advance_t(int64_t* d_t, int64_t k)
@@ -440,10 +630,10 @@ template <typename S>
 __global__ void k_pml_p(PmlArgs a)
 {
     using C = typename Compute<S>::T;
-    const int32_t* bx = a.boxes + 7 * blockIdx.y;
+    // grid: x*y flattened on blockIdx.x, z plane on blockIdx.y, box on blockIdx.z
+    const int32_t* bx = a.boxes + 7 * blockIdx.z;
     const int room = bx[0];
-    const int wx = bx[2] - bx[1], wy = bx[4] - bx[3], wz = bx[6] - bx[5];
-    const int64_t nbox = (int64_t)wx * wy * wz;
+    const int wx = bx[2] - bx[1], wy = bx[4] - bx[3];
     const RoomDev r = a.rooms[room];
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     S* out = (S*)a.out;
@@ -460,10 +650,12 @@ __global__ void k_pml_p(PmlArgs a)
     const C* zy = (const C*)a.z[1] + r.axis_off[1];
     const C* zz = (const C*)a.z[2] + r.axis_off[2];
     const C l2 = (C)a.l2;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += (int64_t)gridDim.x * blockDim.x) {
-        const int x = bx[1] + (int)(q % wx);
-        const int y = bx[3] + (int)((q / wx) % wy);
-        const int z = bx[5] + (int)(q / ((int64_t)wx * wy));
+    {
+        const int rr = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        const int z = bx[5] + (int)blockIdx.y;
+        if (rr >= wx * wy || z >= bx[6]) return;
+        const int x = bx[1] + rr % wx;
+        const int y = bx[3] + rr / wx;
         const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
         const C pc = ld<S, C>(cur, i);
         const C pm = ld<S, C>(prev, i);
@@ -481,10 +673,10 @@ template <typename S>
 __global__ void k_pml_phi(PmlArgs a)
 {
     using C = typename Compute<S>::T;
-    const int32_t* bx = a.boxes + 7 * blockIdx.y;
+    // grid: x*y flattened on blockIdx.x, z plane on blockIdx.y, box on blockIdx.z
+    const int32_t* bx = a.boxes + 7 * blockIdx.z;
     const int room = bx[0];
-    const int wx = bx[2] - bx[1], wy = bx[4] - bx[3], wz = bx[6] - bx[5];
-    const int64_t nbox = (int64_t)wx * wy * wz;
+    const int wx = bx[2] - bx[1], wy = bx[4] - bx[3];
     const RoomDev r = a.rooms[room];
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     const S* p1 = (const S*)a.out;
@@ -501,32 +693,28 @@ __global__ void k_pml_phi(PmlArgs a)
     const C* hz = (const C*)a.zh[2] + r.axis_off[2];
     const C c = (C)a.c, dx = (C)a.dx, dt = (C)a.dt;
     const C cc = c * c, idt = (C)1.0 / dt, idx = (C)1.0 / dx;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nbox; q += (int64_t)gridDim.x * blockDim.x) {
-        const int x = bx[1] + (int)(q % wx);
-        const int y = bx[3] + (int)((q / wx) % wy);
-        const int z = bx[5] + (int)(q / ((int64_t)wx * wy));
+    {
+        const int rr = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+        const int z = bx[5] + (int)blockIdx.y;
+        if (rr >= wx * wy || z >= bx[6]) return;
+        const int x = bx[1] + rr % wx;
+        const int y = bx[3] + rr / wx;
         const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
         const C p1c = ld<S, C>(p1, i), pcc = ld<S, C>(p, i);
         {
             const C h = hx[x];
-            const C g2 = zy[y] + zz[z] - h, g3 = zy[y] * zz[z];
-            const C dn = (ld<S, C>(p1, i + 1) - p1c) * idx, dold = (ld<S, C>(p, i + 1) - pcc) * idx;
-            const C dps = (psi[i + 1] - psi[i]) * idx;
-            phx[i] = ((idt - (C)0.5 * h) * phx[i] + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
+            phx[i] = phi_update<C>(phx[i], p1c, ld<S, C>(p1, i + 1), pcc, ld<S, C>(p, i + 1), psi[i], psi[i + 1], h,
+                                   zy[y] + zz[z] - h, zy[y] * zz[z], cc, idt, idx);
         }
         {
             const C h = hy[y];
-            const C g2 = zx[x] + zz[z] - h, g3 = zx[x] * zz[z];
-            const C dn = (ld<S, C>(p1, i + sy) - p1c) * idx, dold = (ld<S, C>(p, i + sy) - pcc) * idx;
-            const C dps = (psi[i + sy] - psi[i]) * idx;
-            phy[i] = ((idt - (C)0.5 * h) * phy[i] + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
+            phy[i] = phi_update<C>(phy[i], p1c, ld<S, C>(p1, i + sy), pcc, ld<S, C>(p, i + sy), psi[i], psi[i + sy],
+                                   h, zx[x] + zz[z] - h, zx[x] * zz[z], cc, idt, idx);
         }
         {
             const C h = hz[z];
-            const C g2 = zx[x] + zy[y] - h, g3 = zx[x] * zy[y];
-            const C dn = (ld<S, C>(p1, i + sz) - p1c) * idx, dold = (ld<S, C>(p, i + sz) - pcc) * idx;
-            const C dps = (psi[i + sz] - psi[i]) * idx;
-            phz[i] = ((idt - (C)0.5 * h) * phz[i] + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
+            phz[i] = phi_update<C>(phz[i], p1c, ld<S, C>(p1, i + sz), pcc, ld<S, C>(p, i + sz), psi[i], psi[i + sz],
+                                   h, zx[x] + zy[y] - h, zx[x] * zy[y], cc, idt, idx);
         }
     }
 }
@@ -644,13 +832,17 @@ static hipError_t interior_t(const StepArgs& a, hipStream_t s)
 }
 
 template <typename S, int TX, int TY, int TZ>
-static hipError_t fused_t(const StepArgs& a, const int32_t* teo, const int32_t* tep, const int32_t* tso,
-                          hipStream_t s)
+static hipError_t fused_t(const StepArgs& a, hipStream_t s)
 {
     using C = typename Compute<S>::T;
+    constexpr int NT = TX * TY;
     BndCtx<S, C> c = make_ctx<S, C>(a);
-    hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.ntiles), dim3(TX * TY), 0, s, c, a.tiles, teo, tep, tso, a.d_t,
-                       a.t_rel);
+    const int nblk_b = (a.nb + NT - 1) / NT;
+    const int nblk_s = (a.nsp + NT - 1) / NT;
+    const int nblk_p = (a.npml + NT - 1) / NT;
+    const int tile0 = (nblk_b + nblk_s + nblk_p + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles, a.ntiles, a.nb,
+                       nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.mask, a.d_t, a.t_rel);
     return hipGetLastError();
 }
 
@@ -670,17 +862,16 @@ static hipError_t interior_s(const StepArgs& a, hipStream_t s)
 }
 
 template <typename S>
-static hipError_t fused_s(const StepArgs& a, const int32_t* teo, const int32_t* tep, const int32_t* tso,
-                          hipStream_t s)
+static hipError_t fused_s(const StepArgs& a, hipStream_t s)
 {
     switch (a.tile) {
-    case 0: return fused_t<S, 64, 4, 16>(a, teo, tep, tso, s);
-    case 1: return fused_t<S, 64, 4, 32>(a, teo, tep, tso, s);
-    case 2: return fused_t<S, 64, 8, 16>(a, teo, tep, tso, s);
-    case 3: return fused_t<S, 64, 2, 32>(a, teo, tep, tso, s);
-    case 4: return fused_t<S, 128, 2, 16>(a, teo, tep, tso, s);
-    case 5: return fused_t<S, 64, 4, 8>(a, teo, tep, tso, s);
-    case 6: return fused_t<S, 256, 1, 16>(a, teo, tep, tso, s);
+    case 0: return fused_t<S, 64, 4, 16>(a, s);
+    case 1: return fused_t<S, 64, 4, 32>(a, s);
+    case 2: return fused_t<S, 64, 8, 16>(a, s);
+    case 3: return fused_t<S, 64, 2, 32>(a, s);
+    case 4: return fused_t<S, 128, 2, 16>(a, s);
+    case 5: return fused_t<S, 64, 4, 8>(a, s);
+    case 6: return fused_t<S, 256, 1, 16>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -697,14 +888,14 @@ hipError_t launch_interior(const StepArgs& a, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_fused(const StepArgs& a, const int32_t* teo, const int32_t* tep, const int32_t* tso, hipStream_t s)
+hipError_t launch_fused(const StepArgs& a, hipStream_t s)
 {
     if (a.ntiles <= 0) return hipSuccess;
     switch (a.prec) {
-    case Prec::F32: return fused_s<float>(a, teo, tep, tso, s);
-    case Prec::F64: return fused_s<double>(a, teo, tep, tso, s);
-    case Prec::F16: return fused_s<_Float16>(a, teo, tep, tso, s);
-    case Prec::BF16: return fused_s<__bf16>(a, teo, tep, tso, s);
+    case Prec::F32: return fused_s<float>(a, s);
+    case Prec::F64: return fused_s<double>(a, s);
+    case Prec::F16: return fused_s<_Float16>(a, s);
+    case Prec::BF16: return fused_s<__bf16>(a, s);
     }
     return hipErrorInvalidValue;
 }
@@ -742,10 +933,8 @@ hipError_t launch_advance_t(int64_t* d_t, int64_t k, hipStream_t s)
 
 static dim3 pml_grid(const PmlArgs& a)
 {
-    int64_t blocks = (a.box_cells_max + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    if (blocks < 1) blocks = 1;
-    return dim3((unsigned)blocks, (unsigned)a.nbox);
+    const int blocks = (a.box_xy_max + 255) / 256;
+    return dim3((unsigned)(blocks > 0 ? blocks : 1), (unsigned)(a.box_z_max > 0 ? a.box_z_max : 1), (unsigned)a.nbox);
 }
 
 hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s)

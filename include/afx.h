@@ -167,6 +167,11 @@ typedef struct afx_profile {
 } afx_profile;
 int afx_profile_run(afx_sim* sim, int64_t steps, afx_profile* out);
 
+/* Selects one of the compiled interior tile shapes (x, y, z cells per
+ * workgroup): 64x4x16 (default), 64x4x32, 64x8x16, 64x2x32, 128x2x16, 64x4x8,
+ * 256x1x16.  Results do not depend on the shape. */
+int afx_set_tile(afx_sim* sim, int32_t tx, int32_t ty, int32_t tz);
+
 /* Times the candidate interior tile shapes on this handle's rooms (each is
  * first checked bit-exact against the default shape on a copy of the state),
  * keeps the fastest and returns it in tile[3].  Leaves the simulation state
