@@ -48,7 +48,7 @@ class Stats(C.Structure):
         ("corners", C.c_int64), ("group_count", C.c_int64 * 26), ("steps", C.c_int64),
         ("rooms", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
         ("tile_x", C.c_int32), ("tile_y", C.c_int32), ("tile_z", C.c_int32), ("fused", C.c_int32),
-        ("last_run_ms", C.c_double),
+        ("last_run_ms", C.c_double), ("last_run_gpu_ms", C.c_double), ("last_run_steps", C.c_int64),
     ]
 
 

@@ -1,0 +1,59 @@
+"""Batched impulse-response identification: many independent rooms stepped
+together in one handle (one fused launch per step covers every room),
+against the same rooms run one after another.
+
+The reference runs one room per process and one source per run
+(examples/ie224-impulse.py:79-106); every room here carries its own
+source(s) and receivers, and each receiver trace is one impulse response.
+"""
+import time
+
+import numpy as np
+
+from . import _afx
+
+
+def make_batch(workloads, precision="fp32", compat=False, **kw):
+    """One handle holding every workload as its own room (shared c, dx, dt)."""
+    w0 = workloads[0]
+    s = _afx.Sim(c=w0.c, dx=w0.dx, dt=w0.dt, res=w0.dx, precision=precision, compat=compat, **kw)
+    s.set_voxels(w0.vox)
+    s.set_materials(w0.a, w0.b)
+    ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in workloads[1:]]
+    for rid, w in zip(ids, workloads):
+        for cell, sig in zip(w.src_cells, w.src):
+            s.add_source_cell(cell, sig, room=rid)
+    for rid, w in zip(ids, workloads):
+        for cell in w.mic_cells:
+            s.add_receiver_cell(cell, room=rid)
+    return s
+
+
+def run_batched(workloads, steps, precision="fp32", **kw):
+    """Returns (traces [sum of receivers, steps], wall seconds incl. setup)."""
+    t0 = time.perf_counter()
+    s = make_batch(workloads, precision, **kw)
+    s.run(steps)
+    tr = s.receivers()
+    s.close()
+    return tr, time.perf_counter() - t0
+
+
+def run_sequential(workloads, steps, precision="fp32", **kw):
+    """One handle per room, run back to back."""
+    from .rooms import make_sim
+    t0 = time.perf_counter()
+    out = []
+    for w in workloads:
+        s = make_sim(w, _afx.Sim, precision=precision, **kw)
+        s.run(steps)
+        out.append(s.receivers())
+        s.close()
+    return np.concatenate(out, axis=0), time.perf_counter() - t0
+
+
+def rel_l2(a, b):
+    """Relative L2 error of a against b over all traces."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))

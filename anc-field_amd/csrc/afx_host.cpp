@@ -187,11 +187,16 @@ struct afx_sim {
     int graph_steps = 30;
     int64_t group_count[26] = {0};
     double last_run_ms = 0.0;
+    double last_run_gpu_ms = 0.0;
+    int64_t last_run_steps = 0;
+    hipEvent_t ev_run[2] = {nullptr, nullptr};
 
     ~afx_sim()
     {
         for (auto& g : graph)
             if (g) (void)hipGraphExecDestroy(g);
+        for (auto e : ev_run)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void drop_graphs()
@@ -1326,12 +1331,22 @@ int afx_run(afx_sim* s, int64_t steps)
     HIP_TRY(hipSetDevice(s->cfg.device));
     if (int rc = commit(s)) return rc;
     if (steps == 0) return AFX_OK;
+    if (!s->ev_run[0]) {
+        HIP_TRY(hipEventCreate(&s->ev_run[0]));
+        HIP_TRY(hipEventCreate(&s->ev_run[1]));
+    }
     auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(s->ev_run[0], s->stream));
     int rc = run_steps(s, steps);
     if (rc) return rc;
+    HIP_TRY(hipEventRecord(s->ev_run[1], s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     auto t1 = std::chrono::steady_clock::now();
     s->last_run_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    float gms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&gms, s->ev_run[0], s->ev_run[1]));
+    s->last_run_gpu_ms = gms;
+    s->last_run_steps = steps;
     return AFX_OK;
 }
 
@@ -1444,6 +1459,8 @@ int afx_get_stats(afx_sim* s, afx_stats* st)
     st->tile_z = kTiles[s->tile].tz;
     st->fused = use_fused(s) ? 1 : 0;
     st->last_run_ms = s->last_run_ms;
+    st->last_run_gpu_ms = s->last_run_gpu_ms;
+    st->last_run_steps = s->last_run_steps;
     return AFX_OK;
 }
 

@@ -42,6 +42,31 @@ def cpu_baseline(w, budget_s=12.0):
                       f"FP32 oracle (C, OpenMP) on {cores} host threads"}
 
 
+def batched_leg(steps):
+    """Batched identification: 48 seeded rooms (64^3 .. 256x192x128, 2-4 IIR
+    materials, one Ricker source and 8 receivers each) in one handle, against
+    the same 48 rooms run one after another in FP32; reduced-precision
+    storage errors are stated against the FP64 run of the same batch."""
+    from anc_field_py import batch, rooms
+    ws = rooms.batch_rooms(steps=steps)
+    n_ir = sum(len(w.mic_cells) for w in ws)
+    cells = sum(w.cells for w in ws)
+    seq, t_seq = batch.run_sequential(ws, steps, "fp32")
+    res = {"rooms": len(ws), "steps": steps, "impulse_responses": n_ir, "cells": cells,
+           "sequential_fp32": {"ir_per_s": round(n_ir / t_seq, 2), "wall_s": round(t_seq, 3)}}
+    runs = {}
+    for prec in ("fp32", "fp16", "bf16", "fp64"):
+        runs[prec] = batch.run_batched(ws, steps, prec)
+    ref = runs["fp64"][0]
+    for prec, (tr, t) in runs.items():
+        res[f"batched_{prec}"] = {
+            "ir_per_s": round(n_ir / t, 2), "wall_s": round(t, 3), "speedup_vs_sequential_fp32": round(t_seq / t, 2),
+            "gcell_per_s": round(cells * steps / t / 1e9, 1),
+            "rel_l2_vs_fp64": batch.rel_l2(tr, ref)}
+    res["batched_fp32"]["bitexact_vs_sequential"] = bool((runs["fp32"][0] == seq).all())
+    return res
+
+
 def traffic_for(name):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, if one exists for this workload."""
@@ -62,6 +87,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=300)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -86,8 +113,12 @@ def main():
     elapsed = t1 - t0
     gcells = w.cells * args.steps / elapsed / 1e9
     st = sim.stats()
-    prof = sim.profile(args.profile_steps)
-    k_ms = prof.kernel_ms[0]
+    # Dominant kernel: with the fused step kernel there is one launch per step
+    # (plus one 1-thread counter kernel per 30-step graph), so its average
+    # launch duration is the device time of the timed region, taken from HIP
+    # events recorded on the library's stream, over the number of steps.
+    k_ms = st.last_run_gpu_ms / st.last_run_steps
+    prof = sim.profile(args.profile_steps)  # eager launches, one event pair per kernel
     bpc = BYTES_PER_CELL[args.precision]
     achieved = bpc * w.cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     kernel = "k_fused" if st.fused else "k_interior"
@@ -119,6 +150,8 @@ def main():
         "roofline": roof,
     }
     sim.close()
+    if not args.no_batched:
+        out["batched"] = batched_leg(args.batched_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

@@ -94,6 +94,8 @@ typedef struct afx_stats {
     int32_t tile_x, tile_y, tile_z; /* interior tile in use */
     int32_t fused;          /* 1 when the fused step kernel is in use */
     double last_run_ms;     /* wall time of the last afx_run (device-synchronised) */
+    double last_run_gpu_ms; /* device time of the last afx_run: HIP events on the handle's stream */
+    int64_t last_run_steps; /* steps of the last afx_run */
 } afx_stats;
 
 /* configuration with the reference's constants: c=340, dx=0.02, dt=3.125e-5,
