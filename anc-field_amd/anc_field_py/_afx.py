@@ -314,3 +314,129 @@ class Sim:
         t = (C.c_int32 * 3)()
         _check(lib.afx_autotune(self._h, steps_per_trial, t))
         return tuple(t)
+
+
+# ---- surrogate MLP (include/afx_mlp.h) ------------------------------------
+
+class MlpConfig(C.Structure):
+    _fields_ = [
+        ("in_dim", C.c_int32), ("n_freq", C.c_int32), ("width", C.c_int32), ("depth", C.c_int32),
+        ("out_dim", C.c_int32), ("batch", C.c_int32), ("device", C.c_int32), ("seed", C.c_uint32),
+        ("freq_min", C.c_float), ("freq_max", C.c_float),
+        ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("weight_decay", C.c_float),
+    ]
+
+
+class MlpStats(C.Structure):
+    _fields_ = [
+        ("params", C.c_int64), ("train_steps", C.c_int64), ("flop_per_step", C.c_double),
+        ("last_train_ms", C.c_double), ("last_train_gpu_ms", C.c_double),
+        ("last_forward_ms", C.c_double), ("last_forward_gpu_ms", C.c_double), ("last_forward_rows", C.c_int64),
+    ]
+
+
+_MLP_SIGS = {
+    "afx_mlp_config_default": (C.c_int, [C.POINTER(MlpConfig)]),
+    "afx_mlp_create": (C.c_int, [C.POINTER(MlpConfig), C.POINTER(_P)]),
+    "afx_mlp_destroy": (None, [_P]),
+    "afx_mlp_param_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "afx_mlp_get_params": (C.c_int, [_P, _f32p, C.c_int64]),
+    "afx_mlp_set_params": (C.c_int, [_P, _f32p, C.c_int64]),
+    "afx_mlp_get_frequencies": (C.c_int, [_P, _f32p, C.c_int32]),
+    "afx_mlp_forward": (C.c_int, [_P, _f32p, C.c_int64, _f32p]),
+    "afx_mlp_gradients": (C.c_int, [_P, _f32p, _f32p, _f32p, C.c_int64, _f32p, C.POINTER(C.c_double), _f32p]),
+    "afx_mlp_set_data": (C.c_int, [_P, _f32p, _f32p, _f32p, C.c_int64]),
+    "afx_mlp_train": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, C.c_uint32, _f32p]),
+    "afx_mlp_get_stats": (C.c_int, [_P, C.POINTER(MlpStats)]),
+}
+for _name, (_res, _args) in _MLP_SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+EXPORTED = EXPORTED + tuple(_MLP_SIGS)
+
+
+def mlp_default_config():
+    cfg = MlpConfig()
+    _check(lib.afx_mlp_config_default(C.byref(cfg)))
+    return cfg
+
+
+class Mlp:
+    """One surrogate network on one device (BF16 MFMA kernels, FP32 master weights)."""
+
+    def __init__(self, **kw):
+        cfg = mlp_default_config()
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        self.cfg = cfg
+        h = _P()
+        _check(lib.afx_mlp_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        n = C.c_int64(0)
+        _check(lib.afx_mlp_param_count(h, C.byref(n)))
+        self.nparam = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.afx_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self):
+        out = np.empty(self.nparam, np.float32)
+        _check(lib.afx_mlp_get_params(self._h, out.ctypes.data_as(_f32p), out.size))
+        return out
+
+    def set_params(self, p):
+        p, pp = _f32(p)
+        _check(lib.afx_mlp_set_params(self._h, pp, p.size))
+
+    def frequencies(self):
+        out = np.empty(self.cfg.n_freq, np.float32)
+        _check(lib.afx_mlp_get_frequencies(self._h, out.ctypes.data_as(_f32p), out.size))
+        return out
+
+    def forward(self, x):
+        x, xp = _f32(x)
+        rows = x.shape[0]
+        y = np.empty((rows, self.cfg.out_dim), np.float32)
+        _check(lib.afx_mlp_forward(self._h, xp, rows, y.ctypes.data_as(_f32p)))
+        return y
+
+    def gradients(self, x, target, weight):
+        x, xp = _f32(x)
+        t, tp = _f32(target)
+        w, wp = _f32(weight)
+        rows = x.shape[0]
+        g = np.empty(self.nparam, np.float32)
+        y = np.empty((rows, self.cfg.out_dim), np.float32)
+        loss = C.c_double(0.0)
+        _check(lib.afx_mlp_gradients(self._h, xp, tp, wp, rows, g.ctypes.data_as(_f32p), C.byref(loss),
+                                     y.ctypes.data_as(_f32p)))
+        return g, loss.value, y
+
+    def set_data(self, x, target, weight):
+        x, xp = _f32(x)
+        t, tp = _f32(target)
+        w, wp = _f32(weight)
+        _check(lib.afx_mlp_set_data(self._h, xp, tp, wp, x.shape[0]))
+
+    def train(self, steps, lr0, lr1=0.0, seed=0):
+        loss = np.empty(int(steps), np.float32)
+        _check(lib.afx_mlp_train(self._h, int(steps), lr0, lr1, seed, loss.ctypes.data_as(_f32p)))
+        return loss
+
+    def stats(self):
+        st = MlpStats()
+        _check(lib.afx_mlp_get_stats(self._h, C.byref(st)))
+        return st

@@ -30,7 +30,7 @@ DEVICE = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["afx_kernels.hip", "afx_host.cpp", "afx_kerkythea.cpp"]
+SOURCES = ["afx_kernels.hip", "afx_host.cpp", "afx_kerkythea.cpp", "afx_mlp.hip"]
 
 
 def _run(cmd):

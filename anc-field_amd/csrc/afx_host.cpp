@@ -134,6 +134,16 @@ int group_of_mask(uint32_t m)
 
 }  // namespace
 
+namespace afx {
+// Shared with the surrogate's entry points (afx_mlp.hip): one thread-local
+// message behind afx_last_error for the whole library.
+int report_error(int code, const char* msg)
+{
+    g_err = msg;
+    return code;
+}
+}  // namespace afx
+
 struct afx_sim {
     afx_config cfg{};
     Prec prec = Prec::F32;

@@ -1,0 +1,882 @@
+// Surrogate MLP: BF16 MFMA training and inference kernels and the
+// afx_mlp_* C ABI (include/afx_mlp.h).
+//
+// No counterpart in the reference.  The MLP maps (source, receiver)
+// positions in one room to the band-limited impulse response that
+// anc_field_py/ancutil.py:49-79 otherwise obtains from a full FDTD run.
+//
+// Kernels (gfx950, one 64-lane wave per workgroup, 32x32 output tile per
+// wave = 2x2 v_mfma_f32_16x16x32_bf16 blocks, FP32 accumulators):
+//   k_features       Fourier features of the positions -> X [rows][F] and X^T
+//   k_gemm<EPI>      C = A . B^T over K (A [M][K], B [N][K], both K-contiguous)
+//                    with a fused epilogue:
+//                      HIDDEN  z = C + b; keep z (FP32); a = silu(z) -> BF16 row- and column-major
+//                      OUT     y = C + b (FP32)
+//                      LOSS    y = C + b; g = 2 w (y - t) -> BF16 both layouts; per-wave loss partial
+//                      BACK    g = C * silu'(z_prev) -> BF16 both layouts
+//   k_gemm_grouped   all weight gradients of a step in one launch: dW_l = dY_l^T . X_l
+//                    (A = dY_l^T [out][B], B = X_l^T [in][B], contraction over the batch)
+//   k_colsum         bias gradients, db_l[n] = sum_b dY_l[b][n], all layers in one launch
+//   k_adam           AdamW on the FP32 master weights; refreshes the BF16 copies
+//                    W [out][in] (forward operand) and W^T [in][out] (backward operand)
+//
+// Every operand of an MFMA is read K-contiguous: the forward uses X and W,
+// the input gradient dY and W^T, the weight gradient dY^T and X^T.  The
+// epilogues that produce X and dY write both layouts, so no kernel transposes
+// through LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <random>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/afx.h"
+#include "../../include/afx_mlp.h"
+
+namespace afx {
+int report_error(int code, const char* msg);
+}
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxLayers = 16;
+
+enum Epi { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_LOSS = 2, EPI_BACK = 3 };
+
+struct Gemm {
+    const bf16* A;       // [M][lda], K-contiguous
+    const bf16* B;       // [N][ldb], K-contiguous
+    int lda, ldb, M, N, K;
+    const float* bias;   // [N]: HIDDEN, OUT, LOSS
+    float* f32;          // HIDDEN: z, OUT/LOSS: y (either may be null in LOSS)
+    int ldf;
+    bf16* act;           // HIDDEN: silu(z); LOSS/BACK: gradient.  [M][ldact]
+    bf16* actT;          // same values, [N][ldactT] (may be null)
+    int ldact, ldactT;
+    const float* zprev;  // BACK: pre-activation at this output, [M][ldz]
+    int ldz;
+    const float* target; // LOSS: [*][ldt], row rows[m] (or m when rows is null)
+    int ldt;
+    const int* rows;
+    const float* wrow;   // LOSS: per-row weight
+    float* partial;      // LOSS: one loss partial per workgroup
+};
+
+struct GradProblem {
+    const bf16* A;       // dY^T [M=out][lda]
+    const bf16* B;       // X^T  [N=in][ldb]
+    float* out;          // dW [M][N]
+    int lda, ldb, M, N, K;
+};
+
+struct GradTable {
+    GradProblem p[kMaxLayers];
+    int start[kMaxLayers + 1];  // first workgroup of each problem
+    int n;
+};
+
+struct ColsumTable {
+    const bf16* g[kMaxLayers];  // dY^T [rows_l][ld]
+    float* out[kMaxLayers];
+    int rows[kMaxLayers];
+    int start[kMaxLayers + 1];  // first output row of each layer in the flattened range
+    int ld, K, n;
+};
+
+struct AdamTable {
+    int64_t off[kMaxLayers + 1];  // parameter offset of W_l; b_l follows W_l
+    int rows[kMaxLayers], cols[kMaxLayers];
+    bf16* wb[kMaxLayers];         // [rows][cols]
+    bf16* wt[kMaxLayers];         // [cols][rows]
+    int n;
+};
+
+__device__ inline float silu(float z) { return z / (1.0f + expf(-z)); }
+
+__device__ inline float silu_grad(float z)
+{
+    const float s = 1.0f / (1.0f + expf(-z));
+    return s + z * s * (1.0f - s);
+}
+
+// One wave computes the 32x32 block at (m0, n0) of A . B^T.  MFMA operand
+// layout (16x16x32): lane l supplies row (l & 15) of A and row (l & 15) of
+// B^T's transpose, 8 consecutive k at offset 8*(l >> 4); the result block has
+// rows 4*(l >> 4) + e, e < 4, at column (l & 15).  A and B take the same k
+// assignment, so the order of k within a lane does not matter.
+__device__ inline void mma_block(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb, int K,
+                                 int m0, int n0, f32x4 (&c)[2][2])
+{
+    const int lane = threadIdx.x;
+    const int r = lane & 15, kq = (lane >> 4) << 3;
+    const bf16* pa0 = A + (size_t)(m0 + r) * lda + kq;
+    const bf16* pa1 = pa0 + (size_t)16 * lda;
+    const bf16* pb0 = B + (size_t)(n0 + r) * ldb + kq;
+    const bf16* pb1 = pb0 + (size_t)16 * ldb;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) c[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+    for (int k = 0; k < K; k += 32) {
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(pa0 + k);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(pa1 + k);
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(pb0 + k);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(pb1 + k);
+        c[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, c[0][0], 0, 0, 0);
+        c[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, c[0][1], 0, 0, 0);
+        c[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, c[1][0], 0, 0, 0);
+        c[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c[1][1], 0, 0, 0);
+    }
+}
+
+template <int E>
+__global__ __launch_bounds__(64) void k_gemm(Gemm g)
+{
+    const int tiles_n = g.N >> 5;
+    const int m0 = (int)(blockIdx.x / tiles_n) << 5;
+    const int n0 = (int)(blockIdx.x % tiles_n) << 5;
+    f32x4 c[2][2];
+    mma_block(g.A, g.lda, g.B, g.ldb, g.K, m0, n0, c);
+
+    const int lane = threadIdx.x;
+    float loss = 0.0f;
+    for (int i = 0; i < 2; ++i) {
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + 16 * j + (lane & 15);
+            const int mb = m0 + 16 * i + ((lane >> 4) << 2);
+            bf16x4 col;
+            for (int e = 0; e < 4; ++e) {
+                const int m = mb + e;
+                float v = c[i][j][e];
+                if (E == EPI_HIDDEN) {
+                    const float z = v + g.bias[n];
+                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
+                    v = silu(z);
+                } else if (E == EPI_OUT) {
+                    g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+                    continue;
+                } else if (E == EPI_LOSS) {
+                    const float y = v + g.bias[n];
+                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = y;
+                    const int src = g.rows ? g.rows[m] : m;
+                    const float w = g.wrow[src];
+                    const float d = y - g.target[(size_t)src * g.ldt + n];
+                    loss += w * (d * d);
+                    v = 2.0f * w * d;
+                } else {  // EPI_BACK
+                    v = v * silu_grad(g.zprev[(size_t)m * g.ldz + n]);
+                }
+                const bf16 h = (bf16)v;
+                g.act[(size_t)m * g.ldact + n] = h;
+                col[e] = h;
+            }
+            if (E != EPI_OUT && g.actT) *reinterpret_cast<bf16x4*>(g.actT + (size_t)n * g.ldactT + mb) = col;
+        }
+    }
+    if (E == EPI_LOSS) {
+        for (int s = 32; s > 0; s >>= 1) loss += __shfl_xor(loss, s);
+        if (lane == 0) g.partial[blockIdx.x] = loss;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_gemm_grouped(GradTable t)
+{
+    int p = 0;
+    while (p + 1 < t.n && (int)blockIdx.x >= t.start[p + 1]) ++p;
+    const GradProblem& g = t.p[p];
+    const int tile = (int)blockIdx.x - t.start[p];
+    const int tiles_n = g.N >> 5;
+    const int m0 = (tile / tiles_n) << 5, n0 = (tile % tiles_n) << 5;
+    f32x4 c[2][2];
+    mma_block(g.A, g.lda, g.B, g.ldb, g.K, m0, n0, c);
+    const int lane = threadIdx.x;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + 16 * i + ((lane >> 4) << 2) + e;
+                const int n = n0 + 16 * j + (lane & 15);
+                g.out[(size_t)m * g.N + n] = c[i][j][e];
+            }
+}
+
+// Fourier features: thread (b, j, k) writes sin and cos of 2 pi f_k x[b][j]
+// at features j*2F + k and j*2F + F + k, in X [rows][nfeat] and X^T [nfeat][ldt].
+__global__ void k_features(const float* __restrict__ x, const int* __restrict__ rows, int nrows, int in_dim, int nf,
+                           const float* __restrict__ freq, bf16* __restrict__ X, bf16* __restrict__ XT, int ldt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows * in_dim * nf) return;
+    const int k = i % nf;
+    const int j = (i / nf) % in_dim;
+    const int b = i / (nf * in_dim);
+    const int src = rows ? rows[b] : b;
+    const float cyc = freq[k] * x[(size_t)src * in_dim + j];
+    float s, c;
+    sincospif(2.0f * cyc, &s, &c);
+    const int nfeat = 2 * in_dim * nf;
+    const int fs = j * 2 * nf + k, fc = fs + nf;
+    X[(size_t)b * nfeat + fs] = (bf16)s;
+    X[(size_t)b * nfeat + fc] = (bf16)c;
+    if (XT) {
+        XT[(size_t)fs * ldt + b] = (bf16)s;
+        XT[(size_t)fc * ldt + b] = (bf16)c;
+    }
+}
+
+// Bias gradients: one wave per output row of the flattened layer range.
+__global__ __launch_bounds__(256) void k_colsum(ColsumTable t)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= t.start[t.n]) return;
+    int l = 0;
+    while (l + 1 < t.n && row >= t.start[l + 1]) ++l;
+    const int n = row - t.start[l];
+    const bf16* g = t.g[l] + (size_t)n * t.ld;
+    float s = 0.0f;
+    for (int b = lane; b < t.K; b += 64) s += (float)g[b];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) t.out[l][n] = s;
+}
+
+// AdamW over all parameters (lr == 0 with clear == 0 only refreshes the BF16 copies).
+__global__ void k_adam(AdamTable t, float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2,
+                       const float* __restrict__ grad, float lr, float b1, float b2, float eps, float wd, float bc1,
+                       float bc2, int update)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.off[t.n]) return;
+    int l = 0;
+    while (l + 1 < t.n && i >= t.off[l + 1]) ++l;
+    float v = w[i];
+    if (update) {
+        const float gi = grad[i];
+        const float a = b1 * m1[i] + (1.0f - b1) * gi;
+        const float s = b2 * m2[i] + (1.0f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = s;
+        const float mh = a / bc1, vh = s / bc2;
+        v = v - lr * (mh / (sqrtf(vh) + eps) + wd * v);
+        w[i] = v;
+    }
+    const int64_t local = i - t.off[l];
+    const int64_t nw = (int64_t)t.rows[l] * t.cols[l];
+    if (local < nw) {
+        const int r = (int)(local / t.cols[l]), c = (int)(local % t.cols[l]);
+        const bf16 h = (bf16)v;
+        t.wb[l][local] = h;
+        t.wt[l][(size_t)c * t.rows[l] + r] = h;
+    }
+}
+
+#define HIP_CHECK(expr)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) {                                                                          \
+            char b_[512];                                                                                \
+            snprintf(b_, sizeof b_, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return afx::report_error(AFX_E_HIP, b_);                                                     \
+        }                                                                                                \
+    } while (0)
+
+int err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int err(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    return afx::report_error(code, buf);
+}
+
+struct Arena {
+    char* base = nullptr;
+    size_t used = 0, cap = 0;
+    template <typename T>
+    T* take(size_t n)
+    {
+        const size_t bytes = (n * sizeof(T) + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        return p;
+    }
+};
+
+}  // namespace
+
+struct afx_mlp {
+    afx_mlp_config cfg{};
+    int device = 0;
+    int L = 0, F = 0, B = 0;
+    std::vector<int> in, out;       // per layer
+    std::vector<int64_t> off;       // parameter offsets (W_l, b_l follows), size L+1
+    int64_t nparam = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* mem = nullptr;
+    // device buffers
+    float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *g = nullptr, *freq = nullptr;
+    std::vector<bf16*> wb, wt;
+    bf16 *X = nullptr, *XT = nullptr;
+    std::vector<float*> Z;          // hidden pre-activations [B][out_l]
+    std::vector<bf16*> A, AT;       // hidden activations
+    std::vector<bf16*> D, DT;       // output gradients of each layer
+    float* Y = nullptr;             // [B][out]
+    float *xq = nullptr, *tq = nullptr, *wq = nullptr;  // staging for forward/gradients
+    float* partial = nullptr;       // loss partials of one step
+    int nblk_loss = 0;
+    // training set
+    float *xd = nullptr, *td = nullptr, *wd = nullptr;
+    int64_t ndata = 0;
+    int64_t t = 0;                  // optimizer steps taken
+    afx_mlp_stats st{};
+    std::vector<float> freq_host;
+};
+
+namespace {
+
+int gemm_launch(afx_mlp* s, int epi, Gemm g)
+{
+    if (g.M % 32 || g.N % 32 || g.K % 32) return err(AFX_E_ARG, "gemm %dx%dx%d not a multiple of 32", g.M, g.N, g.K);
+    const dim3 grid((unsigned)((g.M / 32) * (g.N / 32))), block(64);
+    switch (epi) {
+    case EPI_HIDDEN: k_gemm<EPI_HIDDEN><<<grid, block, 0, s->stream>>>(g); break;
+    case EPI_OUT: k_gemm<EPI_OUT><<<grid, block, 0, s->stream>>>(g); break;
+    case EPI_LOSS: k_gemm<EPI_LOSS><<<grid, block, 0, s->stream>>>(g); break;
+    default: k_gemm<EPI_BACK><<<grid, block, 0, s->stream>>>(g); break;
+    }
+    HIP_CHECK(hipGetLastError());
+    return AFX_OK;
+}
+
+// Features of `nrows` positions (gathered through rows when not null), then
+// the hidden layers.  With keep set, pre-activations and the column-major
+// copies needed by the backward pass are written.
+int forward_hidden(afx_mlp* s, const float* x, const int* rows, bool keep)
+{
+    const int nthreads = s->B * s->cfg.in_dim * s->cfg.n_freq;
+    k_features<<<(nthreads + 255) / 256, 256, 0, s->stream>>>(x, rows, s->B, s->cfg.in_dim, s->cfg.n_freq, s->freq,
+                                                             s->X, keep ? s->XT : nullptr, s->B);
+    HIP_CHECK(hipGetLastError());
+    for (int l = 0; l + 1 < s->L; ++l) {
+        Gemm g{};
+        g.A = l == 0 ? s->X : s->A[l - 1];
+        g.lda = s->in[l];
+        g.B = s->wb[l];
+        g.ldb = s->in[l];
+        g.M = s->B;
+        g.N = s->out[l];
+        g.K = s->in[l];
+        g.bias = s->w + s->off[l] + (int64_t)s->out[l] * s->in[l];
+        g.f32 = keep ? s->Z[l] : nullptr;
+        g.ldf = s->out[l];
+        g.act = s->A[l];
+        g.ldact = s->out[l];
+        g.actT = keep ? s->AT[l] : nullptr;
+        g.ldactT = s->B;
+        int rc = gemm_launch(s, EPI_HIDDEN, g);
+        if (rc) return rc;
+    }
+    return AFX_OK;
+}
+
+// One forward + backward over the batch in X (rows gathered from x/target/weight).
+int step_grad(afx_mlp* s, const float* x, const float* target, const float* weight, const int* rows, float* y)
+{
+    int rc = forward_hidden(s, x, rows, true);
+    if (rc) return rc;
+    const int L = s->L;
+    {
+        Gemm g{};
+        g.A = L >= 2 ? s->A[L - 2] : s->X;
+        g.lda = s->in[L - 1];
+        g.B = s->wb[L - 1];
+        g.ldb = s->in[L - 1];
+        g.M = s->B;
+        g.N = s->out[L - 1];
+        g.K = s->in[L - 1];
+        g.bias = s->w + s->off[L - 1] + (int64_t)s->out[L - 1] * s->in[L - 1];
+        g.f32 = y;
+        g.ldf = s->out[L - 1];
+        g.act = s->D[L - 1];
+        g.ldact = s->out[L - 1];
+        g.actT = s->DT[L - 1];
+        g.ldactT = s->B;
+        g.target = target;
+        g.ldt = s->out[L - 1];
+        g.rows = rows;
+        g.wrow = weight;
+        g.partial = s->partial;
+        if ((rc = gemm_launch(s, EPI_LOSS, g))) return rc;
+    }
+    for (int l = L - 1; l >= 1; --l) {  // input gradient of layer l = output gradient of layer l-1
+        Gemm g{};
+        g.A = s->D[l];
+        g.lda = s->out[l];
+        g.B = s->wt[l];
+        g.ldb = s->out[l];
+        g.M = s->B;
+        g.N = s->in[l];
+        g.K = s->out[l];
+        g.act = s->D[l - 1];
+        g.ldact = s->in[l];
+        g.actT = s->DT[l - 1];
+        g.ldactT = s->B;
+        g.zprev = s->Z[l - 1];
+        g.ldz = s->in[l];
+        if ((rc = gemm_launch(s, EPI_BACK, g))) return rc;
+    }
+    GradTable gt{};
+    gt.n = L;
+    gt.start[0] = 0;
+    for (int l = 0; l < L; ++l) {
+        GradProblem& p = gt.p[l];
+        p.A = s->DT[l];
+        p.lda = s->B;
+        p.B = l == 0 ? s->XT : s->AT[l - 1];
+        p.ldb = s->B;
+        p.out = s->g + s->off[l];
+        p.M = s->out[l];
+        p.N = s->in[l];
+        p.K = s->B;
+        gt.start[l + 1] = gt.start[l] + (p.M / 32) * (p.N / 32);
+    }
+    k_gemm_grouped<<<gt.start[L], 64, 0, s->stream>>>(gt);
+    HIP_CHECK(hipGetLastError());
+    ColsumTable ct{};
+    ct.n = L;
+    ct.ld = s->B;
+    ct.K = s->B;
+    ct.start[0] = 0;
+    for (int l = 0; l < L; ++l) {
+        ct.g[l] = s->DT[l];
+        ct.out[l] = s->g + s->off[l] + (int64_t)s->out[l] * s->in[l];
+        ct.rows[l] = s->out[l];
+        ct.start[l + 1] = ct.start[l] + s->out[l];
+    }
+    k_colsum<<<(ct.start[L] + 3) / 4, 256, 0, s->stream>>>(ct);
+    HIP_CHECK(hipGetLastError());
+    return AFX_OK;
+}
+
+int adam(afx_mlp* s, float lr, bool update)
+{
+    AdamTable at{};
+    at.n = s->L;
+    for (int l = 0; l <= s->L; ++l) at.off[l] = s->off[l];
+    for (int l = 0; l < s->L; ++l) {
+        at.rows[l] = s->out[l];
+        at.cols[l] = s->in[l];
+        at.wb[l] = s->wb[l];
+        at.wt[l] = s->wt[l];
+    }
+    float bc1 = 1.0f, bc2 = 1.0f;
+    if (update) {
+        s->t += 1;
+        bc1 = (float)(1.0 - std::pow((double)s->cfg.beta1, (double)s->t));
+        bc2 = (float)(1.0 - std::pow((double)s->cfg.beta2, (double)s->t));
+    }
+    const int64_t blocks = (s->nparam + 255) / 256;
+    k_adam<<<(unsigned)blocks, 256, 0, s->stream>>>(at, s->w, s->m1, s->m2, s->g, lr, s->cfg.beta1, s->cfg.beta2,
+                                                    s->cfg.eps, s->cfg.weight_decay, bc1, bc2, update ? 1 : 0);
+    HIP_CHECK(hipGetLastError());
+    return AFX_OK;
+}
+
+int set_device(afx_mlp* s)
+{
+    HIP_CHECK(hipSetDevice(s->device));
+    return AFX_OK;
+}
+
+void init_params(const afx_mlp* s, std::vector<float>& p)
+{
+    std::mt19937_64 rng(s->cfg.seed);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    p.assign((size_t)s->nparam, 0.0f);
+    for (int l = 0; l < s->L; ++l) {
+        // He-uniform for the SiLU layers, a tenth of it for the linear output
+        double a = std::sqrt(6.0 / s->in[l]);
+        if (l == s->L - 1) a *= 0.1;
+        float* W = p.data() + s->off[l];
+        for (int64_t i = 0; i < (int64_t)s->out[l] * s->in[l]; ++i) W[i] = (float)(a * u(rng));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int afx_mlp_config_default(afx_mlp_config* c)
+{
+    if (!c) return err(AFX_E_ARG, "null config");
+    *c = afx_mlp_config{};
+    c->in_dim = 6;
+    c->n_freq = 64;
+    c->width = 512;
+    c->depth = 6;
+    c->out_dim = 1024;
+    c->batch = 256;
+    c->device = 0;
+    c->seed = 1;
+    c->freq_min = 0.25f;
+    c->freq_max = 32.0f;
+    c->beta1 = 0.9f;
+    c->beta2 = 0.999f;
+    c->eps = 1e-8f;
+    c->weight_decay = 0.0f;
+    return AFX_OK;
+}
+
+int afx_mlp_create(const afx_mlp_config* cfg, afx_mlp** out)
+{
+    if (!cfg || !out) return err(AFX_E_ARG, "null argument");
+    *out = nullptr;
+    const afx_mlp_config& c = *cfg;
+    if (c.in_dim < 1 || c.n_freq < 1 || c.depth < 2 || c.depth > kMaxLayers)
+        return err(AFX_E_ARG, "need in_dim >= 1, n_freq >= 1 and 2 <= depth <= %d", kMaxLayers);
+    const int F = 2 * c.in_dim * c.n_freq;
+    if (F % 32 || c.width % 32 || c.out_dim % 32 || c.batch % 32 || c.width < 32 || c.out_dim < 32 || c.batch < 32)
+        return err(AFX_E_ARG, "features (%d), width, out_dim and batch must be positive multiples of 32", F);
+    if ((int64_t)c.batch * std::max(F, std::max(c.width, c.out_dim)) > (int64_t)1 << 30)
+        return err(AFX_E_RANGE, "batch x layer size above 2^30");
+    if (!(c.freq_min > 0) || !(c.freq_max >= c.freq_min)) return err(AFX_E_ARG, "need 0 < freq_min <= freq_max");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return err(AFX_E_HIP, "no HIP device available");
+    if (c.device < 0 || c.device >= n) return err(AFX_E_ARG, "device %d outside 0..%d", c.device, n - 1);
+
+    afx_mlp* s = new (std::nothrow) afx_mlp;
+    if (!s) return err(AFX_E_NOMEM, "out of host memory");
+    s->cfg = c;
+    s->device = c.device;
+    s->L = c.depth;
+    s->F = F;
+    s->B = c.batch;
+    for (int l = 0; l < s->L; ++l) {
+        s->in.push_back(l == 0 ? F : c.width);
+        s->out.push_back(l == s->L - 1 ? c.out_dim : c.width);
+    }
+    s->off.assign(s->L + 1, 0);
+    for (int l = 0; l < s->L; ++l) s->off[l + 1] = s->off[l] + (int64_t)s->out[l] * s->in[l] + s->out[l];
+    s->nparam = s->off[s->L];
+    s->freq_host.resize(c.n_freq);
+    for (int k = 0; k < c.n_freq; ++k) {
+        const double r = c.n_freq > 1 ? (double)k / (c.n_freq - 1) : 0.0;
+        s->freq_host[k] = (float)(c.freq_min * std::pow((double)c.freq_max / c.freq_min, r));
+    }
+
+    auto fail_free = [&](int rc) {
+        afx_mlp_destroy(s);
+        return rc;
+    };
+    if (int rc = set_device(s)) return fail_free(rc);
+    if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
+        hipEventCreate(&s->ev1) != hipSuccess)
+        return fail_free(err(AFX_E_HIP, "stream/event creation failed"));
+
+    const int B = s->B;
+    s->nblk_loss = (B / 32) * (c.out_dim / 32);
+    // sizes of everything, then one allocation
+    Arena probe;
+    auto layout = [&](Arena& a) {
+        s->w = a.take<float>(s->nparam);
+        s->m1 = a.take<float>(s->nparam);
+        s->m2 = a.take<float>(s->nparam);
+        s->g = a.take<float>(s->nparam);
+        s->freq = a.take<float>(c.n_freq);
+        s->wb.assign(s->L, nullptr);
+        s->wt.assign(s->L, nullptr);
+        for (int l = 0; l < s->L; ++l) {
+            s->wb[l] = a.take<bf16>((size_t)s->out[l] * s->in[l]);
+            s->wt[l] = a.take<bf16>((size_t)s->out[l] * s->in[l]);
+        }
+        s->X = a.take<bf16>((size_t)B * F);
+        s->XT = a.take<bf16>((size_t)B * F);
+        s->Z.assign(s->L, nullptr);
+        s->A.assign(s->L, nullptr);
+        s->AT.assign(s->L, nullptr);
+        s->D.assign(s->L, nullptr);
+        s->DT.assign(s->L, nullptr);
+        for (int l = 0; l < s->L; ++l) {
+            if (l + 1 < s->L) {
+                s->Z[l] = a.take<float>((size_t)B * s->out[l]);
+                s->A[l] = a.take<bf16>((size_t)B * s->out[l]);
+                s->AT[l] = a.take<bf16>((size_t)B * s->out[l]);
+            }
+            s->D[l] = a.take<bf16>((size_t)B * s->out[l]);
+            s->DT[l] = a.take<bf16>((size_t)B * s->out[l]);
+        }
+        s->Y = a.take<float>((size_t)B * c.out_dim);
+        s->xq = a.take<float>((size_t)B * c.in_dim);
+        s->tq = a.take<float>((size_t)B * c.out_dim);
+        s->wq = a.take<float>(B);
+        s->partial = a.take<float>(s->nblk_loss);
+    };
+    layout(probe);
+    if (hipMalloc(&s->mem, probe.used) != hipSuccess) return fail_free(err(AFX_E_NOMEM, "device allocation of %zu bytes failed", probe.used));
+    Arena real;
+    real.base = static_cast<char*>(s->mem);
+    real.cap = probe.used;
+    layout(real);
+    if (hipMemsetAsync(s->mem, 0, probe.used, s->stream) != hipSuccess)
+        return fail_free(err(AFX_E_HIP, "clearing device memory failed"));
+    if (hipMemcpyAsync(s->freq, s->freq_host.data(), c.n_freq * sizeof(float), hipMemcpyHostToDevice, s->stream) !=
+        hipSuccess)
+        return fail_free(err(AFX_E_HIP, "frequency upload failed"));
+    std::vector<float> p;
+    init_params(s, p);
+    if (int rc = afx_mlp_set_params(s, p.data(), (int64_t)p.size())) return fail_free(rc);
+    s->st.params = s->nparam;
+    double fwd = 0, bwd = 0;
+    for (int l = 0; l < s->L; ++l) {
+        fwd += 2.0 * B * s->in[l] * s->out[l];
+        bwd += 2.0 * B * s->in[l] * s->out[l] * (l == 0 ? 1 : 2);
+    }
+    s->st.flop_per_step = fwd + bwd;
+    *out = s;
+    return AFX_OK;
+}
+
+void afx_mlp_destroy(afx_mlp* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->mem) (void)hipFree(s->mem);
+    if (s->xd) (void)hipFree(s->xd);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int afx_mlp_param_count(afx_mlp* s, int64_t* n)
+{
+    if (!s || !n) return err(AFX_E_ARG, "null argument");
+    *n = s->nparam;
+    return AFX_OK;
+}
+
+int afx_mlp_get_params(afx_mlp* s, float* out, int64_t cap)
+{
+    if (!s || !out) return err(AFX_E_ARG, "null argument");
+    if (cap < s->nparam) return err(AFX_E_RANGE, "buffer of %lld floats, %lld needed", (long long)cap, (long long)s->nparam);
+    if (int rc = set_device(s)) return rc;
+    HIP_CHECK(hipMemcpyAsync(out, s->w, s->nparam * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    return AFX_OK;
+}
+
+int afx_mlp_set_params(afx_mlp* s, const float* in, int64_t n)
+{
+    if (!s || !in) return err(AFX_E_ARG, "null argument");
+    if (n != s->nparam) return err(AFX_E_ARG, "%lld parameters given, %lld expected", (long long)n, (long long)s->nparam);
+    if (int rc = set_device(s)) return rc;
+    HIP_CHECK(hipMemcpyAsync(s->w, in, n * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemsetAsync(s->m1, 0, n * sizeof(float), s->stream));
+    HIP_CHECK(hipMemsetAsync(s->m2, 0, n * sizeof(float), s->stream));
+    s->t = 0;
+    if (int rc = adam(s, 0.0f, false)) return rc;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    return AFX_OK;
+}
+
+int afx_mlp_get_frequencies(afx_mlp* s, float* out, int32_t cap)
+{
+    if (!s || !out) return err(AFX_E_ARG, "null argument");
+    if (cap < s->cfg.n_freq) return err(AFX_E_RANGE, "buffer of %d floats, %d needed", cap, s->cfg.n_freq);
+    std::memcpy(out, s->freq_host.data(), s->freq_host.size() * sizeof(float));
+    return AFX_OK;
+}
+
+int afx_mlp_forward(afx_mlp* s, const float* x, int64_t rows, float* y)
+{
+    if (!s || !x || !y || rows < 1) return err(AFX_E_ARG, "forward needs x, y and rows >= 1");
+    if (int rc = set_device(s)) return rc;
+    const int B = s->B, I = s->cfg.in_dim, O = s->cfg.out_dim, L = s->L;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_CHECK(hipEventRecord(s->ev0, s->stream));
+    for (int64_t r0 = 0; r0 < rows; r0 += B) {
+        const int64_t nr = std::min<int64_t>(B, rows - r0);
+        if (nr < B) HIP_CHECK(hipMemsetAsync(s->xq, 0, (size_t)B * I * sizeof(float), s->stream));
+        HIP_CHECK(hipMemcpyAsync(s->xq, x + r0 * I, (size_t)nr * I * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        if (int rc = forward_hidden(s, s->xq, nullptr, false)) return rc;
+        Gemm g{};
+        g.A = L >= 2 ? s->A[L - 2] : s->X;
+        g.lda = s->in[L - 1];
+        g.B = s->wb[L - 1];
+        g.ldb = s->in[L - 1];
+        g.M = B;
+        g.N = O;
+        g.K = s->in[L - 1];
+        g.bias = s->w + s->off[L - 1] + (int64_t)O * s->in[L - 1];
+        g.f32 = s->Y;
+        g.ldf = O;
+        if (int rc = gemm_launch(s, EPI_OUT, g)) return rc;
+        HIP_CHECK(hipMemcpyAsync(y + r0 * O, s->Y, (size_t)nr * O * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_CHECK(hipEventRecord(s->ev1, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->st.last_forward_gpu_ms = ms;
+    s->st.last_forward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->st.last_forward_rows = rows;
+    return AFX_OK;
+}
+
+int afx_mlp_gradients(afx_mlp* s, const float* x, const float* target, const float* weight, int64_t rows,
+                      float* grad, double* loss, float* y)
+{
+    if (!s || !x || !target || !weight || !grad) return err(AFX_E_ARG, "null argument");
+    if (rows < 1 || rows > s->B) return err(AFX_E_RANGE, "rows %lld outside 1..%d", (long long)rows, s->B);
+    if (int rc = set_device(s)) return rc;
+    const int B = s->B, I = s->cfg.in_dim, O = s->cfg.out_dim;
+    // padding rows have zero position, target and weight, so they add nothing to loss or gradient
+    HIP_CHECK(hipMemsetAsync(s->xq, 0, (size_t)B * I * sizeof(float), s->stream));
+    HIP_CHECK(hipMemsetAsync(s->tq, 0, (size_t)B * O * sizeof(float), s->stream));
+    HIP_CHECK(hipMemsetAsync(s->wq, 0, (size_t)B * sizeof(float), s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->xq, x, (size_t)rows * I * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->tq, target, (size_t)rows * O * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->wq, weight, (size_t)rows * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    if (int rc = step_grad(s, s->xq, s->tq, s->wq, nullptr, s->Y)) return rc;
+    std::vector<float> part((size_t)s->nblk_loss);
+    HIP_CHECK(hipMemcpyAsync(grad, s->g, s->nparam * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipMemcpyAsync(part.data(), s->partial, part.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    if (y) HIP_CHECK(hipMemcpyAsync(y, s->Y, (size_t)rows * O * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (loss) {
+        double acc = 0.0;
+        for (float v : part) acc += v;
+        *loss = acc;
+    }
+    return AFX_OK;
+}
+
+int afx_mlp_set_data(afx_mlp* s, const float* x, const float* target, const float* weight, int64_t rows)
+{
+    if (!s || !x || !target || !weight) return err(AFX_E_ARG, "null argument");
+    if (rows < s->B) return err(AFX_E_ARG, "training set of %lld rows is smaller than one batch (%d)", (long long)rows, s->B);
+    if (rows > INT32_MAX / 2) return err(AFX_E_RANGE, "training set too large");
+    if (int rc = set_device(s)) return rc;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (s->xd) HIP_CHECK(hipFree(s->xd));
+    s->xd = s->td = s->wd = nullptr;
+    s->ndata = 0;
+    const int I = s->cfg.in_dim, O = s->cfg.out_dim;
+    Arena probe;
+    auto layout = [&](Arena& a) {
+        float* xd = a.take<float>((size_t)rows * I);
+        float* td = a.take<float>((size_t)rows * O);
+        float* wd = a.take<float>((size_t)rows);
+        return std::make_tuple(xd, td, wd);
+    };
+    layout(probe);
+    void* mem = nullptr;
+    if (hipMalloc(&mem, probe.used) != hipSuccess) return err(AFX_E_NOMEM, "device allocation of %zu bytes failed", probe.used);
+    Arena real;
+    real.base = static_cast<char*>(mem);
+    std::tie(s->xd, s->td, s->wd) = layout(real);
+    HIP_CHECK(hipMemcpyAsync(s->xd, x, (size_t)rows * I * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->td, target, (size_t)rows * O * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(s->wd, weight, (size_t)rows * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->ndata = rows;
+    return AFX_OK;
+}
+
+int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed, float* loss_out)
+{
+    if (!s) return err(AFX_E_ARG, "null handle");
+    if (steps < 1) return err(AFX_E_ARG, "steps must be >= 1");
+    if (!s->xd) return err(AFX_E_STATE, "no training set (afx_mlp_set_data)");
+    if (!(lr0 >= 0) || !(lr1 >= 0)) return err(AFX_E_ARG, "learning rates must be >= 0");
+    if (int rc = set_device(s)) return rc;
+    const int B = s->B;
+    // minibatch index stream: a fresh permutation per epoch, consumed B at a time
+    std::vector<int> idx((size_t)steps * B);
+    std::vector<int> perm((size_t)s->ndata);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    size_t pos = perm.size();
+    for (size_t i = 0; i < idx.size(); ++i) {
+        if (pos == perm.size()) {
+            std::shuffle(perm.begin(), perm.end(), rng);
+            pos = 0;
+        }
+        idx[i] = perm[pos++];
+    }
+    int* d_idx = nullptr;
+    float* d_part = nullptr;
+    if (hipMalloc(&d_idx, idx.size() * sizeof(int)) != hipSuccess)
+        return err(AFX_E_NOMEM, "device allocation for the index stream failed");
+    if (hipMalloc(&d_part, (size_t)steps * s->nblk_loss * sizeof(float)) != hipSuccess) {
+        (void)hipFree(d_idx);
+        return err(AFX_E_NOMEM, "device allocation for the loss record failed");
+    }
+    auto cleanup = [&](int rc) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(d_idx);
+        (void)hipFree(d_part);
+        return rc;
+    };
+    if (hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return cleanup(err(AFX_E_HIP, "index upload failed"));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
+    float* keep_partial = s->partial;
+    for (int64_t k = 0; k < steps; ++k) {
+        const double frac = steps > 1 ? (double)k / (double)(steps - 1) : 1.0;
+        const float lr = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(M_PI * frac)));
+        s->partial = d_part + k * s->nblk_loss;
+        int rc = step_grad(s, s->xd, s->td, s->wd, d_idx + k * B, nullptr);
+        if (!rc) rc = adam(s, lr, true);
+        if (rc) {
+            s->partial = keep_partial;
+            return cleanup(rc);
+        }
+    }
+    s->partial = keep_partial;
+    if (hipEventRecord(s->ev1, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "training failed on the device"));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    s->st.last_train_gpu_ms = ms;
+    s->st.last_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->st.train_steps = s->t;
+    if (loss_out) {
+        std::vector<float> part((size_t)steps * s->nblk_loss);
+        if (hipMemcpy(part.data(), d_part, part.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            return cleanup(err(AFX_E_HIP, "loss readback failed"));
+        for (int64_t k = 0; k < steps; ++k) {
+            double acc = 0.0;
+            for (int b = 0; b < s->nblk_loss; ++b) acc += part[(size_t)k * s->nblk_loss + b];
+            loss_out[k] = (float)acc;
+        }
+    }
+    return cleanup(AFX_OK);
+}
+
+int afx_mlp_get_stats(afx_mlp* s, afx_mlp_stats* st)
+{
+    if (!s || !st) return err(AFX_E_ARG, "null argument");
+    *st = s->st;
+    st->train_steps = s->t;
+    return AFX_OK;
+}
+
+}  // extern "C"

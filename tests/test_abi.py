@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every function include/afx.h declares."""
+"""The C-ABI library loads and exports every function include/*.h declares."""
 import ctypes
 import os
 import re
@@ -7,9 +7,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    text = open(os.path.join(ROOT, "include", "afx.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(afx_[a-z_0-9]+)\s*\(", text)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(afx_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
