@@ -392,7 +392,7 @@ int build_special(afx_sim* s)
             if (it != s->entries.end() && it->cell == q.cell) {
                 b = (int32_t)(it - s->entries.begin());
                 info[(size_t)b] |= kInfoSpecial;
-            } else if (s->pml_mode == 2 && q.room == 0) {
+            } else if (s->pml_mode == 2) {
                 // a PML cell: its value (and psi) come from the PML update; the
                 // special pass only adds the sources to the stored value
                 const int w = s->pml_width;
@@ -409,6 +409,7 @@ int build_special(afx_sim* s)
         for (int32_t v : q.mic) mids.push_back(v);
         moff.push_back((int32_t)mids.size());
     }
+    if (cell.size() >= (size_t)(1 << 24) - 2) return fail(AFX_E_RANGE, "%zu source/receiver cells, at most 2^24-3", cell.size());
     s->nsp = (int32_t)cell.size();
     HIP_TRY(upload(s->d_sp_cell, cell, s->stream));
     HIP_TRY(upload(s->d_sp_room, room, s->stream));
@@ -419,13 +420,18 @@ int build_special(afx_sim* s)
     HIP_TRY(upload(s->d_sp_mic, mids, s->stream));
     HIP_TRY(upload(s->d_e_info, info, s->stream));
     HIP_TRY(upload(s->d_sp_xyz, xyz, s->stream));
-    // fused path: every cell of room 0's PML slabs (solid ones included) is
-    // one PML thread; boundary cells there only advance phi and psi, PML
+    // fused path: every cell of each room's PML slabs (solid ones included)
+    // is one PML thread; boundary cells there only advance phi and psi, PML
     // source/receiver cells are finished by their PML thread
     std::vector<int32_t> pcell, pflag;
     std::vector<uint32_t> pxyz;
-    if (s->pml_mode == 2) {
-        const Room& r = s->rooms[0];
+    std::vector<std::pair<int64_t, int32_t>> pml_sp;  // (cell, special index) of PML cells
+    if (s->pml_mode == 2)
+        for (size_t j = 0; j < bidx.size(); ++j)
+            if (bidx[j] == -3) pml_sp.push_back({cell[j], (int32_t)j});
+    std::sort(pml_sp.begin(), pml_sp.end());
+    for (size_t ri = 0; s->pml_mode == 2 && ri < s->rooms.size(); ++ri) {
+        const Room& r = s->rooms[ri];
         const int w = s->pml_width;
         int lo[3], hi[3];
         const int n3[3] = {r.nx, r.ny, r.nz};
@@ -433,10 +439,6 @@ int build_special(afx_sim* s)
             lo[k] = ((s->pml_faces >> (2 * k)) & 1u) ? 1 + w : 0;
             hi[k] = ((s->pml_faces >> (2 * k + 1)) & 1u) ? n3[k] - 1 - w : n3[k];
         }
-        std::vector<std::pair<int64_t, int32_t>> pml_sp;  // (cell, special index) of PML cells
-        for (size_t j = 0; j < bidx.size(); ++j)
-            if (bidx[j] == -3) pml_sp.push_back({cell[j], (int32_t)j});
-        std::sort(pml_sp.begin(), pml_sp.end());
         for (int z = 1; z < r.nz - 1; ++z)
             for (int y = 1; y < r.ny - 1; ++y)
                 for (int x = 1; x < r.nx - 1; ++x) {
@@ -453,7 +455,7 @@ int build_special(afx_sim* s)
                     }
                     pcell.push_back((int32_t)c);
                     pxyz.push_back(pack_xyz(x, y, z));
-                    pflag.push_back(flag);
+                    pflag.push_back((int32_t)(((uint32_t)ri << 24) | (uint32_t)(flag + 2)));
                     mask[(size_t)(c >> 5)] |= 1u << (c & 31);
                 }
     }
@@ -539,7 +541,8 @@ int setup_pml_t(afx_sim* s)
         s->pml_rot = 0;
         return AFX_OK;
     }
-    // physical: graded zeta on the selected faces of room 0
+    // physical: graded zeta on the selected faces of every room; the per-axis
+    // profiles of the rooms are concatenated (RoomDev::axis_off)
     const int w = s->pml_width;
     const double order = s->cfg.pml_order > 0 ? s->cfg.pml_order : 3.0;
     const double r0 = s->cfg.pml_r0 > 0 ? s->cfg.pml_r0 : 1e-3;
@@ -556,50 +559,57 @@ int setup_pml_t(afx_sim* s)
         }
         return zmax * std::pow(std::min(depth, (double)w) / w, order);
     };
-    std::vector<std::array<int, 6>> boxes;
+    std::vector<std::array<int, 7>> boxes;  // room, x0, x1, y0, y1, z0, z1
+    std::vector<C> zc_all[3], zh_all[3];
+    for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+        const Room& rm = s->rooms[ri];
+        const int rdims[3] = {rm.nx, rm.ny, rm.nz};
+        std::vector<std::array<int, 6>> rboxes;
+        for (int d = 0; d < 3; ++d) {
+            const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
+            const int n = rdims[d];
+            for (int i = 0; i <= n; ++i) {
+                double vc = 0, vh = 0;
+                if (i < n && hi) {  // interface at n-1-w-0.5
+                    vc = std::max(vc, prof(i - (n - 1 - w) + 0.5));
+                    vh = std::max(vh, prof(i - (n - 1 - w) + 1.0));
+                }
+                if (i < n && lo) {  // interface at w+0.5
+                    vc = std::max(vc, prof((w + 0.5) - i));
+                    vh = std::max(vh, prof((double)(w - i)));
+                }
+                zc_all[d].push_back((C)vc);
+                zh_all[d].push_back((C)vh);
+            }
+            std::array<int, 6> full = {1, rm.nx - 1, 1, rm.ny - 1, 1, rm.nz - 1};
+            for (int side = 0; side < 2; ++side) {
+                if (!(side ? hi : lo)) continue;
+                std::array<int, 6> slab = full;
+                if (side) slab[2 * d] = n - 1 - w;
+                else slab[2 * d + 1] = 1 + w;
+                std::vector<std::array<int, 6>> pieces = {slab};
+                for (const auto& b : rboxes) {
+                    std::vector<std::array<int, 6>> next;
+                    for (const auto& p : pieces) subtract_box(p, b, next);
+                    pieces.swap(next);
+                }
+                for (const auto& p : pieces)
+                    if (p[1] > p[0] && p[3] > p[2] && p[5] > p[4]) rboxes.push_back(p);
+            }
+        }
+        for (const auto& b : rboxes) boxes.push_back({(int)ri, b[0], b[1], b[2], b[3], b[4], b[5]});
+    }
     for (int d = 0; d < 3; ++d) {
-        std::vector<C> zc((size_t)dims[d] + 1, (C)0), zh((size_t)dims[d] + 1, (C)0);
-        const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
-        const int n = dims[d];
-        for (int i = 0; i < n; ++i) {
-            double vc = 0, vh = 0;
-            if (hi) {  // interface at n-1-w-0.5
-                vc = std::max(vc, prof(i - (n - 1 - w) + 0.5));
-                vh = std::max(vh, prof(i - (n - 1 - w) + 1.0));
-            }
-            if (lo) {  // interface at w+0.5
-                vc = std::max(vc, prof((w + 0.5) - i));
-                vh = std::max(vh, prof((double)(w - i)));
-            }
-            zc[(size_t)i] = (C)vc;
-            zh[(size_t)i] = (C)vh;
-        }
-        HIP_TRY(upload(s->d_z[d], zc, s->stream));
-        HIP_TRY(upload(s->d_zh[d], zh, s->stream));
-        std::array<int, 6> full = {1, r.nx - 1, 1, r.ny - 1, 1, r.nz - 1};
-        for (int side = 0; side < 2; ++side) {
-            if (!(side ? hi : lo)) continue;
-            std::array<int, 6> slab = full;
-            if (side) slab[2 * d] = n - 1 - w;
-            else slab[2 * d + 1] = 1 + w;
-            std::vector<std::array<int, 6>> pieces = {slab};
-            for (const auto& b : boxes) {
-                std::vector<std::array<int, 6>> next;
-                for (const auto& p : pieces) subtract_box(p, b, next);
-                pieces.swap(next);
-            }
-            for (const auto& p : pieces)
-                if (p[1] > p[0] && p[3] > p[2] && p[5] > p[4]) boxes.push_back(p);
-        }
+        HIP_TRY(upload(s->d_z[d], zc_all[d], s->stream));
+        HIP_TRY(upload(s->d_zh[d], zh_all[d], s->stream));
     }
     std::vector<int32_t> bx;
     s->box_xy_max = 0;
     s->box_z_max = 0;
     for (const auto& b : boxes) {
-        bx.push_back(0);
-        for (int k = 0; k < 6; ++k) bx.push_back(b[(size_t)k]);
-        s->box_xy_max = std::max(s->box_xy_max, (b[1] - b[0]) * (b[3] - b[2]));
-        s->box_z_max = std::max(s->box_z_max, b[5] - b[4]);
+        for (int k = 0; k < 7; ++k) bx.push_back(b[(size_t)k]);
+        s->box_xy_max = std::max(s->box_xy_max, (b[2] - b[1]) * (b[4] - b[3]));
+        s->box_z_max = std::max(s->box_z_max, b[6] - b[5]);
     }
     s->nbox = (int32_t)boxes.size();
     HIP_TRY(upload(s->d_boxes, bx, s->stream));
@@ -654,11 +664,10 @@ int commit(afx_sim* s)
     s->total_cells = base;
     if (base >= (int64_t)INT32_MAX - 1) return fail(AFX_E_RANGE, "%lld cells exceed the 2^31 index range", (long long)base);
     if (s->pml_mode == 1 && s->rooms.size() > 1) return fail(AFX_E_ARG, "compat PML supports a single room");
-    if (s->pml_mode == 2) {
-        const Room& r = s->rooms[0];
-        if (2 * s->pml_width + 2 >= std::min(r.nx, std::min(r.ny, r.nz)))
-            return fail(AFX_E_ARG, "PML width %d too large for the %dx%dx%d grid", s->pml_width, r.nx, r.ny, r.nz);
-    }
+    if (s->pml_mode == 2)
+        for (const Room& r : s->rooms)
+            if (2 * s->pml_width + 2 >= std::min(r.nx, std::min(r.ny, r.nz)))
+                return fail(AFX_E_ARG, "PML width %d too large for the %dx%dx%d grid", s->pml_width, r.nx, r.ny, r.nz);
     // boundary entries, sorted by cell
     std::fill(std::begin(s->group_count), std::end(s->group_count), 0);
     s->entries.clear();
@@ -671,6 +680,7 @@ int commit(afx_sim* s)
     s->nb = (int32_t)s->entries.size();
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<RoomDev> rd;
+    int32_t axis_acc[3] = {0, 0, 0};
     for (auto& r : s->rooms) {
         RoomDev d;
         std::memset(&d, 0, sizeof d);
@@ -679,14 +689,16 @@ int commit(afx_sim* s)
         d.ny = r.ny;
         d.nz = r.nz;
         const int dims[3] = {r.nx, r.ny, r.nz};
-        const bool pml = s->pml_mode == 2 && rd.empty();  // the PML belongs to room 0
+        const bool pml = s->pml_mode == 2;  // the physical PML lines the same faces of every room
         d.pml = pml ? 1 : 0;
         for (int k = 0; k < 3; ++k) {
             const bool lo = pml && ((s->pml_faces >> (2 * k)) & 1u);
             const bool hi = pml && ((s->pml_faces >> (2 * k + 1)) & 1u);
             d.pml_lo[k] = lo ? 1 + s->pml_width : 0;
             d.pml_hi[k] = hi ? dims[k] - 1 - s->pml_width : dims[k];
-            d.axis_off[k] = 0;
+            // per-axis zeta profiles of the rooms are concatenated, dims[k] + 1 entries each
+            d.axis_off[k] = pml ? axis_acc[k] : 0;
+            if (pml) axis_acc[k] += dims[k] + 1;
         }
         rd.push_back(d);
     }

@@ -347,7 +347,8 @@ struct BndCtx {
     C cc, idt, idx;
     const int32_t* pml_cell;
     const uint32_t* pml_xyz;
-    const int32_t* pml_flag;  // -2: boundary cell (phi/psi only), -1: plain, >= 0: special-cell index
+    const int32_t* pml_flag;  // bits 24-31 room; bits 0-23 kind + 2, kind -2: boundary cell (phi/psi
+                              // only), -1: plain, >= 0: special-cell index
 };
 
 __device__ __forceinline__ void unpack_xyz(uint32_t p, int& x, int& y, int& z)
@@ -460,7 +461,9 @@ __device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int k, int64_t 
     const C* phr1 = c.phib[par ^ 1][1];
     const C* phr2 = c.phib[par ^ 1][2];
     const C* psr = c.psib[par ^ 1];
-    const RoomDev r = c.rooms[0];
+    const uint32_t word = (uint32_t)c.pml_flag[k];
+    const RoomDev r = c.rooms[word >> 24];
+    const int flag = (int)(word & 0xFFFFFFu) - 2;
     const int64_t i = c.pml_cell[k];
     int x, y, z;
     unpack_xyz(c.pml_xyz[k], x, y, z);
@@ -475,7 +478,6 @@ __device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int k, int64_t 
     const C ps = psr[i];
     const C psn = ps + c.pml.dt * pc;
     c.psib[par][i] = psn;
-    const int flag = c.pml_flag[k];
     if (flag == -2) return;  // boundary cell: its pressure comes from the LRS update
     const C fxm = phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, phr0, psr, 1);
     const C fym = phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, phr1, psr, sy);

@@ -6,7 +6,7 @@ face, band-limited noise source, 16 receivers), FP32, physical boundary
 mode, run through the C ABI.  Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
-                  [--compat] [--no-cpu-baseline] [--tune]
+                  [--compat] [--no-cpu-baseline] [--tune] [--no-batched] [--no-surrogate]
 """
 import argparse
 import json
@@ -67,6 +67,62 @@ def batched_leg(steps):
     return res
 
 
+def surrogate_leg(train_steps):
+    """Learned surrogate in the furnished room (generator seed fixed): 64
+    sources x 64 receivers = 4096 FDTD impulse responses for training and
+    8 x 64 = 512 held out, positions uniform over free cells, each trace
+    low-passed at 2 kHz and decimated by 4 to 1024 taps.  The 6 x 512
+    Fourier-feature MLP (64 frequencies) trains on the BF16 MFMA kernels,
+    batch 256.  Held-out queries go through the validated-bound check and
+    fall back to FDTD, which judges the network's answers."""
+    import numpy as np
+    from anc_field_py import rooms, surrogate
+    w = rooms.furnished(seed=2, steps=4096)
+    rng = np.random.Generator(np.random.PCG64(5))
+    src = surrogate.free_cells(w, rng, 72)
+    rcv = [surrogate.free_cells(w, rng, 64) for _ in range(72)]
+    sur = surrogate.Surrogate(w, seed=1)
+    try:
+        t = time.perf_counter()
+        ir = sur.simulate(src, rcv)
+        t_fdtd = time.perf_counter() - t
+        ps, pr = surrogate.pairs(src[:64], rcv[:64])
+        hs, hr = surrogate.pairs(src[64:], rcv[64:])
+        tr_ir, ho_ir = ir[:len(ps)], ir[len(ps):]
+        loss = sur.fit(sur.encode(ps, pr), tr_ir, train_steps, 1e-3, 1e-5, seed=3)
+        st = sur.net.stats()
+        e_tr = sur.validate(ps, pr, tr_ir)
+        fwd = sur.net.stats()
+        e_ho = surrogate.rel_l2(sur.predict(hs, hr), ho_ir)
+        e_nn = surrogate.rel_l2(surrogate.nearest_pair_baseline(ps, pr, tr_ir, hs, hr), ho_ir)
+        runs = sur.fdtd_runs
+        t = time.perf_counter()
+        ans, fb, _ = sur.query(np.concatenate([ps[:512], hs]), np.concatenate([pr[:512], hr]))
+        t_query = time.perf_counter() - t
+        truth = np.concatenate([tr_ir[:512], ho_ir])
+        e_ans = surrogate.rel_l2(ans, truth)
+        train_s = st.last_train_gpu_ms * 1e-3
+        return {
+            "train_pairs": int(len(ps)), "heldout_pairs": int(len(hs)), "taps": sur.taps, "bound": sur.bound,
+            "fdtd_impulse_responses_per_s": round(len(ir) / t_fdtd, 1), "fdtd_s": round(t_fdtd, 2),
+            "net": {"layers": int(sur.net.cfg.depth), "width": int(sur.net.cfg.width), "fourier_freqs":
+                    int(sur.net.cfg.n_freq), "params": int(st.params), "batch": int(sur.net.cfg.batch)},
+            "train_steps": int(train_steps), "train_gpu_s": round(train_s, 3),
+            "train_tflops": round(st.flop_per_step * train_steps / train_s / 1e12, 2),
+            "mfma_bf16_peak_tflops": 2500.0,
+            "loss_first_100": float(np.mean(loss[:100])), "loss_last_100": float(np.mean(loss[-100:])),
+            "train_rel_l2_median": float(np.median(e_tr)), "train_within_bound": float(np.mean(e_tr <= sur.bound)),
+            "heldout_rel_l2_median": float(np.median(e_ho)), "heldout_within_bound": float(np.mean(e_ho <= sur.bound)),
+            "nearest_training_pair_heldout_rel_l2_median": float(np.median(e_nn)),
+            "net_forward_impulse_responses_per_s": round(fwd.last_forward_rows / (fwd.last_forward_ms * 1e-3), 1),
+            "query": {"pairs": int(len(ans)), "answered_by_net": int((~fb).sum()), "fdtd_fallbacks": int(fb.sum()),
+                      "fdtd_source_runs": int(sur.fdtd_runs - runs), "wall_s": round(t_query, 3),
+                      "max_rel_l2_vs_fdtd": float(np.max(e_ans))},
+        }
+    finally:
+        sur.close()
+
+
 def traffic_for(name):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, if one exists for this workload."""
@@ -89,6 +145,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=4096)
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=30000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -152,6 +210,8 @@ def main():
     sim.close()
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_steps)
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_leg(args.surrogate_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

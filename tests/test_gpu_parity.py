@@ -218,6 +218,59 @@ def test_batched_rooms_match_single():
         k += len(w.mic_cells)
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_batched_rooms_with_pml_match_single(fused):
+    """Every room of a handle gets the PML; each room equals its own run, with
+    sources and receivers inside the PML slabs too."""
+    _case(f"batched rooms with physical PML vs one handle per room, fused={fused}")
+    ws = rooms.batch_rooms(seed=8, nrooms=3, steps=150, lo=(26, 22, 20), hi=(40, 36, 30), nmic=3)
+    faces = afx.FACE_XP | afx.FACE_YM
+    width = 4
+    extra = []  # one source and one receiver cell inside each room's x+ slab
+    for w in ws:
+        nz, ny, nx = w.vox.shape
+        extra.append((np.array([nx - 3, ny // 2, nz // 2], np.int32), np.array([nx - 2, ny // 3, nz // 3], np.int32)))
+
+    def setup(s, rid, w, ex):
+        for cell, sig in zip(w.src_cells, w.src):
+            s.add_source_cell(cell, sig, room=rid)
+        s.add_source_cell(ex[0], 0.5 * w.src[0], room=rid)
+
+    def mics(s, rid, w, ex):
+        for cell in w.mic_cells:
+            s.add_receiver_cell(cell, room=rid)
+        s.add_receiver_cell(ex[1], room=rid)
+
+    single = []
+    for w, ex in zip(ws, extra):
+        s = afx.Sim(c=w.c, dx=w.dx, dt=w.dt, res=w.dx, compat=False, fused=fused)
+        s.set_voxels(w.vox)
+        s.set_materials(w.a, w.b)
+        s.set_pml(width, faces)
+        setup(s, 0, w, ex)
+        mics(s, 0, w, ex)
+        s.run(w.steps)
+        single.append(s.receivers())
+        s.close()
+    s = afx.Sim(c=ws[0].c, dx=ws[0].dx, dt=ws[0].dt, res=ws[0].dx, compat=False, fused=fused)
+    s.set_voxels(ws[0].vox)
+    s.set_materials(ws[0].a, ws[0].b)
+    s.set_pml(width, faces)
+    ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
+    for rid, w, ex in zip(ids, ws, extra):
+        setup(s, rid, w, ex)
+    for rid, w, ex in zip(ids, ws, extra):
+        mics(s, rid, w, ex)
+    s.run(ws[0].steps)
+    tr = s.receivers()
+    s.close()
+    k = 0
+    for w, want in zip(ws, single):
+        assert np.abs(want[-1]).max() > 0  # the PML receiver sees the field
+        _assert_bitexact(tr[k:k + len(want)], want, w.name)
+        k += len(want)
+
+
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
 def test_reduced_precision_error(precision):
     _case(f"reduced precision {precision}")

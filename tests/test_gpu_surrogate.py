@@ -1,0 +1,89 @@
+"""Surrogate workflow on a small room: batched FDTD impulse responses ->
+BF16 MFMA training -> validation -> queries with FDTD fallback.
+
+Room: the 64x48x32 shoebox with an 8-cell PML (rooms.shoebox_small); 128
+taps after a 2 kHz low-pass and decimation by 4 (512 steps); 16 sources x
+16 receivers for training, 4 x 16 held out.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TAPS, DECIM = 128, 4
+NET = dict(n_freq=16, width=128, depth=4, batch=64, seed=3, freq_min=0.25, freq_max=8.0)
+
+
+@pytest.fixture(scope="module")
+def data():
+    print("\n[case] surrogate data: shoebox_small, 20 sources x 16 receivers, 512 steps", flush=True)
+    from anc_field_py import rooms, surrogate
+    w = rooms.shoebox_small(seed=1, steps=TAPS * DECIM)
+    rng = np.random.Generator(np.random.PCG64(11))
+    src = surrogate.free_cells(w, rng, 20)
+    rcv = [surrogate.free_cells(w, rng, 16) for _ in range(20)]
+    sur = surrogate.Surrogate(w, taps=TAPS, decim=DECIM, rooms_per_handle=8, **NET)
+    ir = sur.simulate(src, rcv)
+    yield w, sur, src, rcv, ir
+    sur.close()
+
+
+def test_simulate_matches_single_room_run(data):
+    from anc_field_py import _afx, surrogate
+    w, sur, src, rcv, ir = data
+    s = _afx.Sim(c=w.c, dx=w.dx, dt=w.dt, res=w.dx, compat=False)
+    s.set_voxels(w.vox)
+    s.set_materials(w.a, w.b)
+    s.set_pml(w.pml_width, w.pml_faces, w.pml_profile)
+    s.add_source_cell(src[3], np.array([1.0], np.float32))
+    for c in rcv[3]:
+        s.add_receiver_cell(c)
+    s.run(TAPS * DECIM)
+    want = surrogate.lowpass_decimate(s.receivers(), w.dt, TAPS, DECIM, 2000.0)
+    s.close()
+    assert np.array_equal(ir[3 * 16:4 * 16], want)
+    assert np.all(np.linalg.norm(want, axis=1) > 0)
+
+
+def test_lowpass_decimate_keeps_passband_and_drops_stopband():
+    from anc_field_py import surrogate
+    dt = 1.0 / 32000
+    n = np.arange(4096)
+    lo = np.sin(2 * np.pi * 500 * n * dt)
+    hi = np.sin(2 * np.pi * 6000 * n * dt)
+    y = surrogate.lowpass_decimate(np.stack([lo, hi]), dt, 1000, 4, 2000.0)
+    mid = slice(100, 900)
+    assert np.allclose(y[0, mid], lo[::4][mid], atol=5e-3)
+    assert np.abs(y[1, mid]).max() < 1e-2
+
+
+def test_training_validation_and_fallback(data):
+    print("\n[case] surrogate training 3000 steps + queries", flush=True)
+    from anc_field_py import surrogate
+    w, sur, src, rcv, ir = data
+    ps, pr = surrogate.pairs(src[:16], rcv[:16])
+    hs, hr = surrogate.pairs(src[16:], rcv[16:])
+    t_ir, h_ir = ir[:len(ps)], ir[len(ps):]
+    loss = sur.fit(sur.encode(ps, pr), t_ir, 3000, 2e-3, 2e-5, seed=1)
+    assert loss[-100:].mean() < 0.05 * loss[:20].mean()
+    e_tr = sur.validate(ps, pr, t_ir)
+    e_ho = surrogate.rel_l2(sur.predict(hs, hr), h_ir)
+    ok = e_tr <= sur.bound
+    print(f"training pairs: median rel L2 {np.median(e_tr):.3e}, within 5e-2: {ok.mean():.2f}; "
+          f"held-out median {np.median(e_ho):.3e}")
+    assert np.median(e_tr) < 0.1
+    assert ok.sum() >= len(ok) // 4
+    # queries: validated training pairs and unseen pairs, mixed
+    pick = np.concatenate([np.flatnonzero(ok)[:24], np.flatnonzero(~ok)[:8]])
+    qs = np.concatenate([ps[pick], hs[:16]])
+    qr = np.concatenate([pr[pick], hr[:16]])
+    truth = np.concatenate([t_ir[pick], h_ir[:16]])
+    runs = sur.fdtd_runs
+    out, fb, err = sur.query(qs, qr)
+    n_ok = min(24, int(ok.sum()))
+    assert not fb[:n_ok].any() and fb[n_ok:].all()
+    # every answer is within the bound of the FDTD response; fallbacks are the FDTD response
+    assert np.all(surrogate.rel_l2(out, truth) <= sur.bound)
+    assert np.array_equal(out[fb], truth[fb])
+    assert np.all(np.isfinite(err[fb])) and np.all(np.isnan(err[~fb]))
+    assert sur.fdtd_runs - runs == len({tuple(s) for s in qs[fb]})
