@@ -25,6 +25,7 @@ PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16}
 COMPAT = 1
 NO_GRAPH = 2
 SPLIT = 4
+LIST_BLOCKS = 8
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = 1, 2, 4, 8, 16, 32
 
 
@@ -165,7 +166,7 @@ class Sim:
     """One simulation handle (one or more rooms stepped together)."""
 
     def __init__(self, c=None, dx=None, dt=None, res=None, precision="fp32", compat=True, device=0,
-                 graph=True, fused=True, pml_order=None, pml_r0=None, scale=None):
+                 graph=True, fused=True, pml_order=None, pml_r0=None, scale=None, list_blocks=False):
         cfg = default_config()
         if c is not None:
             cfg.c = c
@@ -176,7 +177,8 @@ class Sim:
         if res is not None:
             cfg.res = res
         cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        cfg.flags = (COMPAT if compat else 0) | (0 if graph else NO_GRAPH) | (0 if fused else SPLIT)
+        cfg.flags = ((COMPAT if compat else 0) | (0 if graph else NO_GRAPH) | (0 if fused else SPLIT)
+                     | (LIST_BLOCKS if list_blocks else 0))
         cfg.device = device
         if pml_order is not None:
             cfg.pml_order = pml_order

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <map>
 #include <tuple>
+#include <unordered_map>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -173,6 +174,8 @@ struct afx_sim {
     DBuf P[3];
     int rot[3] = {0, 1, 2};
     DBuf d_rooms, d_tiles, d_mask, d_e_xyz, d_sp_xyz, d_pml_cell, d_pml_xyz, d_pml_flag;
+    DBuf d_work, d_e_pack, d_pml_nbd;
+    int32_t nwork = 0;
     int32_t npml = 0;
     int32_t ntiles = 0;
     DBuf d_e_cell, d_e_info, d_e_mat, d_ring_in, d_ring_out, d_A, d_B;
@@ -458,6 +461,126 @@ int build_special(afx_sim* s)
                     pflag.push_back((int32_t)(((uint32_t)ri << 24) | (uint32_t)(flag + 2)));
                     mask[(size_t)(c >> 5)] |= 1u << (c & 31);
                 }
+    }
+    // Work table of the fused kernel.  Boundary entries (through an index
+    // list; the IIR rings stay in entry order) and PML cells (the list itself;
+    // its state lives in full-grid arrays) are grouped by the interior tile
+    // that holds them, in XCD-major tile order.  Workgroup b runs on XCD
+    // b % 8, so each XCD's sequence is its tiles in order, each followed by
+    // the list chunks whose last cell lies in it; sequences are interleaved
+    // and padded with idle items.
+    {
+        std::vector<int4> table;
+        std::vector<int32_t> los;
+        build_tiles(s, table, los);
+        const TileShape ts = kTiles[s->tile];
+        const int NT = ts.tx * ts.ty;
+        const size_t nslot = table.size();
+        std::vector<int32_t> slot_of(std::max<size_t>(nslot, 1), -1);
+        for (size_t i = 0; i < los.size(); ++i)
+            if (los[i] >= 0) slot_of[(size_t)los[i]] = (int32_t)i;
+        std::vector<int64_t> roff(s->rooms.size() + 1, 0);
+        for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+            const Room& r = s->rooms[ri];
+            roff[ri + 1] = roff[ri] + (int64_t)((r.nx + ts.tx - 1) / ts.tx) * ((r.ny + ts.ty - 1) / ts.ty) *
+                                          ((r.nz + ts.tz - 1) / ts.tz);
+        }
+        // rank of a slot in XCD-major order: XCD slot % 8, then slot / 8
+        const int64_t per = (int64_t)nslot / 8;
+        auto rank_of_cell = [&](int room, int x, int y, int z) -> int64_t {
+            const Room& r = s->rooms[(size_t)room];
+            const int64_t ntx = (r.nx + ts.tx - 1) / ts.tx, nty = (r.ny + ts.ty - 1) / ts.ty;
+            const int64_t L = roff[(size_t)room] + ((int64_t)(z / ts.tz) * nty + y / ts.ty) * ntx + x / ts.tx;
+            const int64_t sl = slot_of[(size_t)L];
+            return (sl % 8) * per + sl / 8;
+        };
+        std::vector<int32_t> bcount(nslot + 1, 0), pcount(nslot + 1, 0);
+        std::vector<int32_t> brank(s->entries.size()), prank(pcell.size());
+        for (size_t e = 0; e < s->entries.size(); ++e) {
+            const Entry& en = s->entries[e];
+            brank[e] = (int32_t)rank_of_cell(en.room, en.x, en.y, en.z);
+            bcount[(size_t)brank[e] + 1]++;
+        }
+        for (size_t k = 0; k < pcell.size(); ++k) {
+            const int x = (int)(pxyz[k] & 2047u), y = (int)((pxyz[k] >> 11) & 2047u), z = (int)(pxyz[k] >> 22);
+            prank[k] = (int32_t)rank_of_cell((int)((uint32_t)pflag[k] >> 24), x, y, z);
+            pcount[(size_t)prank[k] + 1]++;
+        }
+        for (size_t i = 0; i < nslot; ++i) {
+            bcount[i + 1] += bcount[i];
+            pcount[i + 1] += pcount[i];
+        }
+        std::vector<int32_t> tb(s->entries.size());
+        std::vector<int32_t> fill(bcount.begin(), bcount.end() - 1);
+        for (size_t e = 0; e < s->entries.size(); ++e) tb[(size_t)fill[(size_t)brank[e]]++] = (int32_t)e;
+        std::vector<int32_t> pc2(pcell.size()), pf2(pcell.size());
+        std::vector<uint32_t> px2(pcell.size());
+        std::vector<int32_t> pfill(pcount.begin(), pcount.end() - 1);
+        for (size_t k = 0; k < pcell.size(); ++k) {
+            const size_t d = (size_t)pfill[(size_t)prank[k]]++;
+            pc2[d] = pcell[k];
+            px2[d] = pxyz[k];
+            pf2[d] = pflag[k];
+        }
+        pcell.swap(pc2);
+        pxyz.swap(px2);
+        pflag.swap(pf2);
+        std::vector<std::vector<int4>> seq(8);
+        const int nsp = (int)cell.size();
+        for (int j = 0, q = 0; j < nsp; j += NT, ++q) seq[(size_t)(q % 8)].push_back(make_int4(3, j, std::min(j + NT, nsp), 0));
+        for (int x = 0; x < 8; ++x) {
+            auto& v = seq[(size_t)x];
+            int32_t bcur = bcount[(size_t)(x * per)], pcur = pcount[(size_t)(x * per)];
+            const int32_t bend = bcount[(size_t)((x + 1) * per)], pend = pcount[(size_t)((x + 1) * per)];
+            for (int64_t k = 0; k < per; ++k) {
+                const int64_t sl = k * 8 + x;
+                if (table[(size_t)sl].x >= 0) v.push_back(make_int4(0, (int)sl, 0, 0));
+                const int64_t rk = x * per + k;
+                for (; bcount[(size_t)rk + 1] - bcur >= NT; bcur += NT) v.push_back(make_int4(1, bcur, bcur + NT, 0));
+                for (; pcount[(size_t)rk + 1] - pcur >= NT; pcur += NT) v.push_back(make_int4(2, pcur, pcur + NT, 0));
+            }
+            if (bcur < bend) v.push_back(make_int4(1, bcur, bend, 0));
+            if (pcur < pend) v.push_back(make_int4(2, pcur, pend, 0));
+        }
+        // packed boundary records in work order
+        std::vector<int4> epack(tb.size());
+        for (size_t k = 0; k < tb.size(); ++k) {
+            const Entry& en = s->entries[(size_t)tb[k]];
+            epack[k] = make_int4((int)en.cell, (int)info[(size_t)tb[k]], (int)en.mat, tb[k]);  // with kInfoSpecial
+        }
+        // in-chunk distances to each PML cell's -x, -y, -z neighbours
+        std::vector<int32_t> chunk_of(pcell.size(), -1);
+        for (const auto& v : seq)
+            for (const int4& w : v)
+                if (w.x == 2)
+                    for (int q = w.y; q < w.z; ++q) chunk_of[(size_t)q] = w.y;
+        std::unordered_map<int64_t, int32_t> pos_of;
+        pos_of.reserve(pcell.size() * 2);
+        for (size_t k = 0; k < pcell.size(); ++k) pos_of[pcell[k]] = (int32_t)k;
+        std::vector<uint32_t> nbd(pcell.size(), 0u);
+        for (size_t k = 0; k < pcell.size(); ++k) {
+            const Room& r = s->rooms[(size_t)((uint32_t)pflag[k] >> 24)];
+            const int64_t st3[3] = {1, r.nx, (int64_t)r.nx * r.ny};
+            uint32_t word = 0;
+            for (int d = 0; d < 3; ++d) {
+                auto it = pos_of.find((int64_t)pcell[k] - st3[d]);
+                if (it == pos_of.end()) continue;
+                const int64_t dist = (int64_t)k - it->second;
+                if (chunk_of[(size_t)it->second] == chunk_of[k] && dist > 0 && dist < 1024)
+                    word |= (uint32_t)dist << (10 * d);
+            }
+            nbd[k] = word;
+        }
+        HIP_TRY(upload(s->d_e_pack, epack, s->stream));
+        HIP_TRY(upload(s->d_pml_nbd, nbd, s->stream));
+        size_t len = 0;
+        for (const auto& v : seq) len = std::max(len, v.size());
+        std::vector<int4> work(std::max<size_t>(8 * len, 1), make_int4(-1, 0, 0, 0));
+        for (size_t x = 0; x < 8; ++x)
+            for (size_t k = 0; k < seq[x].size(); ++k) work[8 * k + x] = seq[x][k];
+        s->nwork = (int32_t)(8 * len);
+        HIP_TRY(upload(s->d_work, work, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
     }
     s->npml = (int32_t)pcell.size();
     HIP_TRY(upload(s->d_pml_cell, pcell, s->stream));
@@ -848,6 +971,11 @@ StepArgs make_args(afx_sim* s)
     a.pml_xyz = s->d_pml_xyz.as<uint32_t>();
     a.pml_flag = s->d_pml_flag.as<int32_t>();
     a.npml = s->pml_mode == 2 ? s->npml : 0;
+    const bool cosched = !(s->cfg.flags & AFX_LIST_BLOCKS) && s->nwork > 0;
+    a.work = cosched ? s->d_work.as<int4>() : nullptr;
+    a.nwork = cosched ? s->nwork : 0;
+    a.e_pack = s->d_e_pack.as<int4>();
+    a.pml_nbd = s->d_pml_nbd.as<uint32_t>();
     a.mask = s->d_mask.as<uint32_t>();
     a.e_xyz = s->d_e_xyz.as<uint32_t>();
     a.sp_xyz = s->d_sp_xyz.as<uint32_t>();

@@ -103,6 +103,16 @@ struct StepArgs {
     const uint32_t* pml_xyz;
     const int32_t* pml_flag;
     int32_t npml;
+    // fused path work table (kind, a, b) per workgroup, XCD-interleaved:
+    // interior tiles with the list chunks holding their cells (boundary
+    // entries as packed records {cell, info, mats, entry} grouped by tile;
+    // the PML list, itself grouped by tile, with the in-chunk distances of
+    // each cell's -x/-y/-z neighbours; special cells); null runs the lists
+    // in leading workgroups
+    const int4* work;
+    int32_t nwork;
+    const int4* e_pack;
+    const uint32_t* pml_nbd;
     double dt, dx, c;
 };
 

@@ -371,6 +371,20 @@ __device__ __forceinline__ void boundary_entry(const BndCtx<S, C>& c, int e, uns
     st<S, C>(c.out, i, v);
 }
 
+// Boundary entry from its packed work-order record {cell, info, mats, entry}.
+template <typename S, typename C>
+__device__ __forceinline__ void boundary_packed(const BndCtx<S, C>& c, int4 en, unsigned ixb)
+{
+    const uint32_t info = (uint32_t)en.y;
+    if (info & kInfoSpecial) return;
+    const RoomDev r = c.rooms[info >> kInfoRoomShift];
+    const int64_t i = en.x;
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    const C v = bnd_value<S, C>(en.w, i, info, (uint32_t)en.z, c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+                                c.compat, sy, sz, c.k);
+    st<S, C>(c.out, i, v);
+}
+
 // Adds special cell j's sources (in order) to its new value v, stores it and
 // records its receivers (source.cl:32-40 then mic.cl:32-41).
 template <typename S, typename C>
@@ -497,6 +511,71 @@ __device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int k, int64_t 
         st<S, C>(c.out, i, v);
 }
 
+// PML cells [a, b) of one work chunk, one per thread of the workgroup.  Each
+// thread computes phi^n at its own cell and shares it through LDS; the -x,
+// -y and -z neighbours' phi^n is read from there when the neighbour is in the
+// chunk (nbd[q]: list-index distances, 10 bits each, 0 = not in the chunk)
+// and recomputed otherwise.  Values are those of pml_entry.
+template <typename S, typename C, int NT>
+__device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, int64_t t,
+                                          const uint32_t* __restrict__ nbd, C (*lds)[NT])
+{
+    const int tid = (int)threadIdx.x;
+    const int q = a + tid;
+    const bool valid = q < b;
+    const int par = (int)(t & 1);
+    const C* phr0 = c.phib[par ^ 1][0];
+    const C* phr1 = c.phib[par ^ 1][1];
+    const C* phr2 = c.phib[par ^ 1][2];
+    const C* psr = c.psib[par ^ 1];
+    RoomDev r{};
+    int flag = -2, x = 0, y = 0, z = 0;
+    int64_t i = 0;
+    C fx = 0, fy = 0, fz = 0, pc = 0, ps = 0, psn = 0;
+    if (valid) {
+        const uint32_t word = (uint32_t)c.pml_flag[q];
+        r = c.rooms[word >> 24];
+        flag = (int)(word & 0xFFFFFFu) - 2;
+        i = c.pml_cell[q];
+        unpack_xyz(c.pml_xyz[q], x, y, z);
+        const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+        fx = phi_at<S, C>(c, r, 0, i, x, y, z, phr0, psr, 1);
+        fy = phi_at<S, C>(c, r, 1, i, x, y, z, phr1, psr, sy);
+        fz = phi_at<S, C>(c, r, 2, i, x, y, z, phr2, psr, sz);
+        c.phib[par][0][i] = fx;
+        c.phib[par][1][i] = fy;
+        c.phib[par][2][i] = fz;
+        pc = ld<S, C>(c.cur, i);
+        ps = psr[i];
+        psn = ps + c.pml.dt * pc;
+        c.psib[par][i] = psn;
+        lds[0][tid] = fx;
+        lds[1][tid] = fy;
+        lds[2][tid] = fz;
+    }
+    __syncthreads();
+    if (!valid || flag == -2) return;  // boundary cells: pressure from the LRS update
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    const uint32_t nb = nbd[q];
+    const int dxm = (int)(nb & 1023u), dym = (int)((nb >> 10) & 1023u), dzm = (int)(nb >> 20);
+    const C fxm = dxm ? lds[0][tid - dxm] : phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, phr0, psr, 1);
+    const C fym = dym ? lds[1][tid - dym] : phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, phr1, psr, sy);
+    const C fzm = dzm ? lds[2][tid - dzm] : phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, phr2, psr, sz);
+    const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
+    C sum = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
+    sum = sum + ld<S, C>(c.cur, i + sy);
+    sum = sum + ld<S, C>(c.cur, i - sy);
+    sum = sum + ld<S, C>(c.cur, i + sz);
+    sum = sum + ld<S, C>(c.cur, i - sz);
+    const C v = pml_formula<C>(sum, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
+                               c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt, c.pml.dx,
+                               c.k.l2);
+    if (flag >= 0)
+        finish_special<S, C>(c, flag, i, t, rt<S, C>(v));
+    else
+        st<S, C>(c.out, i, v);
+}
+
 template <typename S, typename C>
 __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
 {
@@ -590,11 +669,45 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
                                                    const int4* __restrict__ tiles, int ntiles, int nb, int nblk_b,
                                                    int nsp, int nblk_s, int npml, int nblk_p, int tile0,
                                                    const uint32_t* __restrict__ mask,
-                                                   const int64_t* __restrict__ d_t, int t_rel)
+                                                   const int64_t* __restrict__ d_t, int t_rel,
+                                                   const int4* __restrict__ work,
+                                                   const int4* __restrict__ e_pack,
+                                                   const uint32_t* __restrict__ pml_nbd)
 {
     using C = typename Compute<S>::T;
     constexpr int NT = TX * TY;
-    const int bid = (int)blockIdx.x;
+    __shared__ C pml_lds[3][NT];
+    int bid = (int)blockIdx.x;
+    if (work) {
+        // Work table: item (kind, a, b).  kind 0: interior tile slot a;
+        // 1: packed boundary entries e_pack[a..b); 2: PML list [a..b);
+        // 3: special cells [a..b); -1: idle.  Items of one XCD (blockIdx % 8) follow
+        // its tiles in order, each list chunk right after the tile holding
+        // its last cell, so both read the same planes of p^n through L2.
+        const int4 wk = work[bid];
+        if (wk.x == 0) {
+            const int4 t4 = tiles[wk.y];
+            const RoomDev r = c.rooms[t4.x];
+            const int x = t4.y + (int)(threadIdx.x % TX);
+            const int y = t4.z + (int)(threadIdx.x / TX);
+            interior_column<S, C, TZ, false, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml, mask);
+            return;
+        }
+        if (wk.x < 0) return;
+        const int64_t t = *d_t + t_rel;
+        if (wk.x == 2) {  // whole workgroup: it meets a barrier
+            pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, pml_lds);
+            return;
+        }
+        const int q = wk.y + (int)threadIdx.x;
+        if (q >= wk.z) return;
+        const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
+        if (wk.x == 1)
+            boundary_packed<S, C>(c, e_pack[q], ixb);
+        else
+            special_entry<S, C>(c, q, ixb, t, 3);
+        return;
+    }
     if (bid >= tile0) {
         const int4 t4 = tiles[bid - tile0];
         if (t4.x < 0) return;
@@ -839,12 +952,18 @@ static hipError_t fused_t(const StepArgs& a, hipStream_t s)
     using C = typename Compute<S>::T;
     constexpr int NT = TX * TY;
     BndCtx<S, C> c = make_ctx<S, C>(a);
+    if (a.work) {
+        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.ntiles, a.nb, 0,
+                           a.nsp, 0, a.npml, 0, 0, a.mask, a.d_t, a.t_rel, a.work, a.e_pack, a.pml_nbd);
+        return hipGetLastError();
+    }
     const int nblk_b = (a.nb + NT - 1) / NT;
     const int nblk_s = (a.nsp + NT - 1) / NT;
     const int nblk_p = (a.npml + NT - 1) / NT;
     const int tile0 = (nblk_b + nblk_s + nblk_p + 7) / 8 * 8;
     hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles, a.ntiles, a.nb,
-                       nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.mask, a.d_t, a.t_rel);
+                       nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.mask, a.d_t, a.t_rel, nullptr, nullptr,
+                       nullptr);
     return hipGetLastError();
 }
 

@@ -59,6 +59,8 @@ extern "C" {
                                corner ring stride ncoef/2, pml.cl over the whole interior) */
 #define AFX_NO_GRAPH 2u     /* launch every step eagerly instead of replaying a captured hipGraph */
 #define AFX_SPLIT 4u        /* separate interior and boundary launches (no fused step kernel) */
+#define AFX_LIST_BLOCKS 8u  /* fused kernel: boundary and PML cells in workgroups of their own instead of
+                               in the tile workgroups that hold them (same results) */
 
 /* PML faces */
 #define AFX_FACE_XM 1u

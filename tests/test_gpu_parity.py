@@ -110,7 +110,8 @@ def test_room_bitexact(compat, ncoef):
     _assert_bitexact(f, wf, "field")
 
 
-@pytest.mark.parametrize("flags", [dict(fused=False), dict(graph=False), dict(fused=False, graph=False)])
+@pytest.mark.parametrize("flags", [dict(fused=False), dict(graph=False), dict(fused=False, graph=False),
+                                   dict(list_blocks=True)])
 def test_launch_variants_bitexact(flags):
     _case(f"launch variants {flags}")
     v, a, b, rng = _closed_room(21)
@@ -134,9 +135,9 @@ def test_resumable_runs():
     _assert_bitexact(f, wf, "field")
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_physical_pml_bitexact(fused):
-    _case(f"physical PML x+ and y- faces fused={fused}")
+@pytest.mark.parametrize("fused,list_blocks", [(True, False), (True, True), (False, False)])
+def test_physical_pml_bitexact(fused, list_blocks):
+    _case(f"physical PML x+ and y- faces fused={fused} list_blocks={list_blocks}")
     v, a, b, rng = _closed_room(23, dims=(48, 40, 30))
     v[:, :, -1] = -1   # open x+ face
     v[:, 0, :] = -1    # open y- face
@@ -153,7 +154,7 @@ def test_physical_pml_bitexact(fused):
                     rooms.ricker(steps, dt, 1300.0, amp=0.3)])
     prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
     got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, pml=(6, 2 | 4, prof),
-                                 lib_kw=dict(fused=fused))
+                                 lib_kw=dict(fused=fused, list_blocks=list_blocks))
     _assert_bitexact(got, want, "traces")
     _assert_bitexact(f, wf, "field")
 
