@@ -94,6 +94,7 @@ _SIGS = {
     "afx_autotune": (C.c_int, [_P, C.c_int32, _i32p]),
     "afx_profile_run": (C.c_int, [_P, C.c_int64, C.POINTER(Profile)]),
     "afx_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "afx_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)
@@ -129,6 +130,13 @@ def device_name(i):
     buf = C.create_string_buffer(256)
     _check(lib.afx_device_name(i, buf, 256))
     return buf.value.decode()
+
+
+def copy_probe(nbytes, reps=20, device=0):
+    """Device copy rate in GB/s (read + write) of a plain copy of nbytes."""
+    g = C.c_double(0.0)
+    _check(lib.afx_copy_probe(device, int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def voxelize_kerkythea(path, res=0.02):

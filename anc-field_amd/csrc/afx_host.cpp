@@ -173,7 +173,7 @@ struct afx_sim {
     int64_t total_cells = 0;
     DBuf P[3];
     int rot[3] = {0, 1, 2};
-    DBuf d_rooms, d_tiles, d_mask, d_e_xyz, d_sp_xyz, d_pml_cell, d_pml_xyz, d_pml_flag;
+    DBuf d_rooms, d_tiles, d_mask, d_e_xyz, d_sp_xyz, d_pml_rec, d_pml_side;
     DBuf d_work, d_e_pack, d_pml_nbd;
     int32_t nwork = 0;
     int32_t npml = 0;
@@ -329,6 +329,60 @@ int build_tile_tables(afx_sim* s)
 }
 
 uint32_t pack_xyz(int x, int y, int z) { return (uint32_t)x | ((uint32_t)y << 11) | ((uint32_t)z << 22); }
+
+// Disjoint boxes covering the union of the PML slabs: each slab minus the
+// boxes already placed.
+void subtract_box(const std::array<int, 6>& a, const std::array<int, 6>& b, std::vector<std::array<int, 6>>& out)
+{
+    // a minus b, boxes half-open [x0,x1) [y0,y1) [z0,z1)
+    if (a[0] >= b[1] || a[1] <= b[0] || a[2] >= b[3] || a[3] <= b[2] || a[4] >= b[5] || a[5] <= b[4]) {
+        out.push_back(a);
+        return;
+    }
+    std::array<int, 6> r = a;
+    if (r[0] < b[0]) { auto c = r; c[1] = b[0]; out.push_back(c); r[0] = b[0]; }
+    if (r[1] > b[1]) { auto c = r; c[0] = b[1]; out.push_back(c); r[1] = b[1]; }
+    if (r[2] < b[2]) { auto c = r; c[3] = b[2]; out.push_back(c); r[2] = b[2]; }
+    if (r[3] > b[3]) { auto c = r; c[2] = b[3]; out.push_back(c); r[3] = b[3]; }
+    if (r[4] < b[4]) { auto c = r; c[5] = b[4]; out.push_back(c); r[4] = b[4]; }
+    if (r[5] > b[5]) { auto c = r; c[4] = b[5]; out.push_back(c); r[5] = b[5]; }
+}
+
+// Disjoint boxes {room, x0, x1, y0, y1, z0, z1} covering each room's
+// physical PML region (the union of its face slabs, cells 1..n-2 on every
+// axis), room by room: each slab minus the boxes already placed.
+std::vector<std::array<int, 7>> pml_box_list(const afx_sim* s)
+{
+    std::vector<std::array<int, 7>> boxes;
+    if (s->pml_mode != 2) return boxes;
+    const int w = s->pml_width;
+    for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+        const Room& rm = s->rooms[ri];
+        const int rdims[3] = {rm.nx, rm.ny, rm.nz};
+        std::vector<std::array<int, 6>> rboxes;
+        for (int d = 0; d < 3; ++d) {
+            const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
+            const int n = rdims[d];
+            const std::array<int, 6> full = {1, rm.nx - 1, 1, rm.ny - 1, 1, rm.nz - 1};
+            for (int side = 0; side < 2; ++side) {
+                if (!(side ? hi : lo)) continue;
+                std::array<int, 6> slab = full;
+                if (side) slab[2 * d] = n - 1 - w;
+                else slab[2 * d + 1] = 1 + w;
+                std::vector<std::array<int, 6>> pieces = {slab};
+                for (const auto& b : rboxes) {
+                    std::vector<std::array<int, 6>> next;
+                    for (const auto& p : pieces) subtract_box(p, b, next);
+                    pieces.swap(next);
+                }
+                for (const auto& p : pieces)
+                    if (p[1] > p[0] && p[3] > p[2] && p[5] > p[4]) rboxes.push_back(p);
+            }
+        }
+        for (const auto& b : rboxes) boxes.push_back({(int)ri, b[0], b[1], b[2], b[3], b[4], b[5]});
+    }
+    return boxes;
+}
 
 // Cells holding sources or receivers.  Each is finished by one thread that
 // computes the step value, adds the sources in order and records receivers.
@@ -582,10 +636,52 @@ int build_special(afx_sim* s)
         HIP_TRY(upload(s->d_work, work, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
+    // one record per PML cell: {cell, packed xyz, room << 24 | kind + 2, box}
+    {
+        const auto boxes = pml_box_list(s);
+        std::vector<size_t> first(s->rooms.size() + 1, boxes.size());  // boxes of a room are contiguous
+        for (size_t b = boxes.size(); b-- > 0;) first[(size_t)boxes[b][0]] = b;
+        std::vector<int64_t> base(boxes.size() + 1, 0);  // compact offsets, as setup_pml_t lays them out
+        for (size_t b = 0; b < boxes.size(); ++b)
+            base[b + 1] = base[b] + (int64_t)(boxes[b][2] - boxes[b][1]) * (boxes[b][4] - boxes[b][3]) *
+                                        (boxes[b][6] - boxes[b][5]);
+        std::vector<int4> rec(pcell.size());
+        std::vector<uint32_t> side(pcell.size());
+        for (size_t k = 0; k < pcell.size(); ++k) {
+            const int room = (int)((uint32_t)pflag[k] >> 24);
+            const int x = (int)(pxyz[k] & 2047u), y = (int)((pxyz[k] >> 11) & 2047u), z = (int)(pxyz[k] >> 22);
+            int box = -1;
+            for (size_t b = first[(size_t)room]; b < boxes.size() && boxes[b][0] == room && box < 0; ++b) {
+                const auto& q = boxes[b];
+                if (x >= q[1] && x < q[2] && y >= q[3] && y < q[4] && z >= q[5] && z < q[6]) box = (int)b;
+            }
+            if (box < 0) return fail(AFX_E_STATE, "PML cell (%d,%d,%d) of room %d outside the PML boxes", x, y, z, room);
+            const auto& q = boxes[(size_t)box];
+            const int wx = q[2] - q[1], wy = q[4] - q[3];
+            const int64_t kc = base[(size_t)box] + ((int64_t)(z - q[5]) * wy + (y - q[3])) * wx + (x - q[1]);
+            uint32_t nb = (uint32_t)wx | ((uint32_t)wy << 11);
+            uint32_t other = 0;
+            const int dd[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+            for (int dir = 0; dir < 6; ++dir) {
+                const int xn = x + dd[dir][0], yn = y + dd[dir][1], zn = z + dd[dir][2];
+                if (xn >= q[1] && xn < q[2] && yn >= q[3] && yn < q[4] && zn >= q[5] && zn < q[6]) {
+                    nb |= 1u << (22 + dir);
+                    continue;
+                }
+                for (size_t b = first[(size_t)room]; b < boxes.size() && boxes[b][0] == room; ++b) {
+                    const auto& o = boxes[b];
+                    if (xn >= o[1] && xn < o[2] && yn >= o[3] && yn < o[4] && zn >= o[5] && zn < o[6]) other |= 1u << dir;
+                }
+            }
+            if (other) nb |= 1u << 28;
+            rec[k] = make_int4((int)pxyz[k], pflag[k], (int)kc, (int)nb);
+            side[k] = (uint32_t)box | (other << 26);
+        }
+        HIP_TRY(upload(s->d_pml_rec, rec, s->stream));
+        HIP_TRY(upload(s->d_pml_side, side, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
     s->npml = (int32_t)pcell.size();
-    HIP_TRY(upload(s->d_pml_cell, pcell, s->stream));
-    HIP_TRY(upload(s->d_pml_xyz, pxyz, s->stream));
-    HIP_TRY(upload(s->d_pml_flag, pflag, s->stream));
     HIP_TRY(upload(s->d_mask, mask, s->stream));
     // sources
     int64_t cap = 1;
@@ -615,24 +711,6 @@ int build_special(afx_sim* s)
     s->dirty_special = false;
     s->drop_graphs();
     return AFX_OK;
-}
-
-// Disjoint boxes covering the union of the PML slabs: each slab minus the
-// boxes already placed.
-void subtract_box(const std::array<int, 6>& a, const std::array<int, 6>& b, std::vector<std::array<int, 6>>& out)
-{
-    // a minus b, boxes half-open [x0,x1) [y0,y1) [z0,z1)
-    if (a[0] >= b[1] || a[1] <= b[0] || a[2] >= b[3] || a[3] <= b[2] || a[4] >= b[5] || a[5] <= b[4]) {
-        out.push_back(a);
-        return;
-    }
-    std::array<int, 6> r = a;
-    if (r[0] < b[0]) { auto c = r; c[1] = b[0]; out.push_back(c); r[0] = b[0]; }
-    if (r[1] > b[1]) { auto c = r; c[0] = b[1]; out.push_back(c); r[1] = b[1]; }
-    if (r[2] < b[2]) { auto c = r; c[3] = b[2]; out.push_back(c); r[2] = b[2]; }
-    if (r[3] > b[3]) { auto c = r; c[2] = b[3]; out.push_back(c); r[3] = b[3]; }
-    if (r[4] < b[4]) { auto c = r; c[5] = b[4]; out.push_back(c); r[4] = b[4]; }
-    if (r[5] > b[5]) { auto c = r; c[4] = b[5]; out.push_back(c); r[5] = b[5]; }
 }
 
 template <typename C>
@@ -682,12 +760,10 @@ int setup_pml_t(afx_sim* s)
         }
         return zmax * std::pow(std::min(depth, (double)w) / w, order);
     };
-    std::vector<std::array<int, 7>> boxes;  // room, x0, x1, y0, y1, z0, z1
     std::vector<C> zc_all[3], zh_all[3];
     for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
         const Room& rm = s->rooms[ri];
         const int rdims[3] = {rm.nx, rm.ny, rm.nz};
-        std::vector<std::array<int, 6>> rboxes;
         for (int d = 0; d < 3; ++d) {
             const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
             const int n = rdims[d];
@@ -704,50 +780,42 @@ int setup_pml_t(afx_sim* s)
                 zc_all[d].push_back((C)vc);
                 zh_all[d].push_back((C)vh);
             }
-            std::array<int, 6> full = {1, rm.nx - 1, 1, rm.ny - 1, 1, rm.nz - 1};
-            for (int side = 0; side < 2; ++side) {
-                if (!(side ? hi : lo)) continue;
-                std::array<int, 6> slab = full;
-                if (side) slab[2 * d] = n - 1 - w;
-                else slab[2 * d + 1] = 1 + w;
-                std::vector<std::array<int, 6>> pieces = {slab};
-                for (const auto& b : rboxes) {
-                    std::vector<std::array<int, 6>> next;
-                    for (const auto& p : pieces) subtract_box(p, b, next);
-                    pieces.swap(next);
-                }
-                for (const auto& p : pieces)
-                    if (p[1] > p[0] && p[3] > p[2] && p[5] > p[4]) rboxes.push_back(p);
-            }
         }
-        for (const auto& b : rboxes) boxes.push_back({(int)ri, b[0], b[1], b[2], b[3], b[4], b[5]});
     }
     for (int d = 0; d < 3; ++d) {
         HIP_TRY(upload(s->d_z[d], zc_all[d], s->stream));
         HIP_TRY(upload(s->d_zh[d], zh_all[d], s->stream));
     }
+    // phi and psi are stored for the PML region only, box after box, each
+    // box x-fastest from its base
+    const auto boxes = pml_box_list(s);
     std::vector<int32_t> bx;
     s->box_xy_max = 0;
     s->box_z_max = 0;
+    int64_t total = 0;
     for (const auto& b : boxes) {
         for (int k = 0; k < 7; ++k) bx.push_back(b[(size_t)k]);
+        bx.push_back((int32_t)total);
+        total += (int64_t)(b[2] - b[1]) * (b[4] - b[3]) * (b[6] - b[5]);
         s->box_xy_max = std::max(s->box_xy_max, (b[2] - b[1]) * (b[4] - b[3]));
         s->box_z_max = std::max(s->box_z_max, b[6] - b[5]);
     }
+    if (total >= (int64_t)INT32_MAX) return fail(AFX_E_RANGE, "PML region of %lld cells above 2^31", (long long)total);
     s->nbox = (int32_t)boxes.size();
     HIP_TRY(upload(s->d_boxes, bx, s->stream));
+    const size_t pbytes = sizeof(C) * (size_t)std::max<int64_t>(total, 1);
     // two sets: the fused path alternates them by step parity, the split
     // path updates set 0 in place
     for (int d = 0; d < 3; ++d) {
-        HIP_TRY(s->d_phi[d].alloc(cbytes));
-        HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)cbytes, s->stream));
-        HIP_TRY(s->d_phi_old[d].alloc(cbytes));
-        HIP_TRY(launch_fill(s->d_phi_old[d].p, (int64_t)cbytes, s->stream));
+        HIP_TRY(s->d_phi[d].alloc(pbytes));
+        HIP_TRY(launch_fill(s->d_phi[d].p, (int64_t)pbytes, s->stream));
+        HIP_TRY(s->d_phi_old[d].alloc(pbytes));
+        HIP_TRY(launch_fill(s->d_phi_old[d].p, (int64_t)pbytes, s->stream));
     }
-    HIP_TRY(s->d_psi.alloc(cbytes));
-    HIP_TRY(launch_fill(s->d_psi.p, (int64_t)cbytes, s->stream));
-    HIP_TRY(s->d_psi_old.alloc(cbytes));
-    HIP_TRY(launch_fill(s->d_psi_old.p, (int64_t)cbytes, s->stream));
+    HIP_TRY(s->d_psi.alloc(pbytes));
+    HIP_TRY(launch_fill(s->d_psi.p, (int64_t)pbytes, s->stream));
+    HIP_TRY(s->d_psi_old.alloc(pbytes));
+    HIP_TRY(launch_fill(s->d_psi_old.p, (int64_t)pbytes, s->stream));
     return AFX_OK;
 }
 
@@ -804,6 +872,7 @@ int commit(afx_sim* s)
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<RoomDev> rd;
     int32_t axis_acc[3] = {0, 0, 0};
+    const auto pboxes = pml_box_list(s);
     for (auto& r : s->rooms) {
         RoomDev d;
         std::memset(&d, 0, sizeof d);
@@ -823,6 +892,12 @@ int commit(afx_sim* s)
             d.axis_off[k] = pml ? axis_acc[k] : 0;
             if (pml) axis_acc[k] += dims[k] + 1;
         }
+        d.box0 = d.box1 = 0;
+        for (size_t b = 0; b < pboxes.size(); ++b)
+            if (pboxes[b][0] == (int)rd.size()) {
+                if (d.box1 == d.box0) d.box0 = (int32_t)b;
+                d.box1 = (int32_t)b + 1;
+            }
         rd.push_back(d);
     }
     HIP_TRY(upload(s->d_rooms, rd, s->stream));
@@ -967,9 +1042,9 @@ StepArgs make_args(afx_sim* s)
     }
     a.pml_psi[0] = s->d_psi.p;
     a.pml_psi[1] = s->d_psi_old.p;
-    a.pml_cell = s->d_pml_cell.as<int32_t>();
-    a.pml_xyz = s->d_pml_xyz.as<uint32_t>();
-    a.pml_flag = s->d_pml_flag.as<int32_t>();
+    a.pml_rec = s->d_pml_rec.as<int4>();
+    a.pml_side = s->d_pml_side.as<uint32_t>();
+    a.pml_boxes = s->d_boxes.as<int32_t>();
     a.npml = s->pml_mode == 2 ? s->npml : 0;
     const bool cosched = !(s->cfg.flags & AFX_LIST_BLOCKS) && s->nwork > 0;
     a.work = cosched ? s->d_work.as<int4>() : nullptr;
@@ -1222,6 +1297,37 @@ int afx_device_name(int32_t i, char* buf, size_t len)
 }
 
 const char* afx_last_error(void) { return g_err.c_str(); }
+
+int afx_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps)
+{
+    if (!gbps || bytes < 16 || bytes % 16 || reps < 1) return fail(AFX_E_ARG, "need bytes a positive multiple of 16, reps >= 1");
+    HIP_TRY(hipSetDevice(device));
+    DBuf a, b;
+    HIP_TRY(a.alloc((size_t)bytes));
+    HIP_TRY(b.alloc((size_t)bytes));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreate(&st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int rc) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+        return rc;
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return done(fail(AFX_E_HIP, "event creation failed"));
+    if (launch_fill(a.p, bytes, st) != hipSuccess || launch_copy(a.p, b.p, bytes / 16, st) != hipSuccess)
+        return done(fail(AFX_E_HIP, "copy probe launch failed"));
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+        if (launch_copy(a.p, b.p, bytes / 16, st) != hipSuccess) return done(fail(AFX_E_HIP, "copy probe launch failed"));
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) return done(fail(AFX_E_HIP, "copy probe failed on the device"));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return done(AFX_OK);
+}
 
 int afx_create(const afx_config* cfg, afx_sim** out)
 {

@@ -15,7 +15,7 @@ struct RoomDev {
     // a cell is in the PML slabs when x < pml_lo[0] or x >= pml_hi[0] (likewise y, z)
     int32_t pml_lo[3], pml_hi[3];
     int32_t axis_off[3]; // offset of this room's per-axis zeta profiles
-    int32_t pad;
+    int32_t box0, box1;  // this room's boxes in the physical PML box table
 };
 
 // Boundary entry info word:
@@ -99,9 +99,9 @@ struct StepArgs {
     void* pml_psi[2];
     const void* pml_z[3];
     const void* pml_zh[3];
-    const int32_t* pml_cell;
-    const uint32_t* pml_xyz;
-    const int32_t* pml_flag;
+    const int4* pml_rec;        // {packed xyz, room << 24 | kind + 2, compact index, box extent/neighbour bits}
+    const uint32_t* pml_side;   // box | neighbours in other boxes << 26, for cells flagged in pml_rec
+    const int32_t* pml_boxes;   // [nbox][8]: room, x0, x1, y0, y1, z0, z1, compact base
     int32_t npml;
     // fused path work table (kind, a, b) per workgroup, XCD-interleaved:
     // interior tiles with the list chunks holding their cells (boundary
@@ -120,13 +120,13 @@ struct StepArgs {
 struct PmlArgs {
     Prec prec;
     void* out; const void* cur; const void* prev;
-    void* phi[3];               // compute type, full grid (physical) / new fi (compat)
+    void* phi[3];               // compute type: compact PML region (physical) / full-grid new fi (compat)
     void* phi_old[3];           // compat: previous fi
     void* psi;                  // physical psi / compat psiph
     void* psi_old;              // compat psimh
     const void* z[3];           // zeta at cell centres per axis (compute type), global per room offset
     const void* zh[3];          // zeta at +1/2 points
-    const int32_t* boxes;       // [nbox][7]: room, x0, x1, y0, y1, z0, z1
+    const int32_t* boxes;       // [nbox][8]: room, x0, x1, y0, y1, z0, z1, compact base of phi/psi
     int32_t nbox;
     int32_t box_xy_max;         // largest box x*y extent (grid sizing)
     int32_t box_z_max;          // largest box z extent
@@ -145,6 +145,7 @@ hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s);
 hipError_t launch_pml_phys_phi(const PmlArgs& a, hipStream_t s);
 hipError_t launch_pml_compat(const PmlArgs& a, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t s);
 hipError_t launch_to_f32(const void* src, Prec prec, float* dst, int64_t n, double scale, hipStream_t s);
 
 }  // namespace afx

@@ -32,8 +32,9 @@ template <typename S, typename C>
 __device__ __forceinline__ C rt(C v) { return static_cast<C>(static_cast<S>(v)); }
 
 // ---------------------------------------------------------------- PML value
-// Physical PML state (compute type C).  phi_x lives at x+1/2 and is stored at
-// index x (likewise y, z); psi is cell-centred at half steps.
+// Physical PML state (compute type C).  phi_x lives at x+1/2 and is stored
+// with cell x (likewise y, z); psi is cell-centred at half steps.  phi and psi
+// are compact: only the PML region is stored, box by box (pml_find).
 template <typename C> struct PmlDev {
     C* phx;
     C* phy;
@@ -62,15 +63,34 @@ __device__ __forceinline__ C pml_formula(C sum, C pc, C pm, C dv, C ps, C psn, C
 }
 
 // Split path: phi^n and psi read from single buffers, psi advanced in place.
+// k is the cell's compact index, kxm/kym/kzm those of its -x/-y/-z
+// neighbours (-1 outside the PML region, where phi is zero).
 template <typename C>
-__device__ __forceinline__ C pml_value(C sum, C pc, C pm, int64_t i, int64_t sy, int64_t sz, C sx, C syy, C szz,
-                                       const PmlDev<C>& q, C l2)
+__device__ __forceinline__ C pml_value(C sum, C pc, C pm, int64_t k, int64_t kxm, int64_t kym, int64_t kzm, C sx,
+                                       C syy, C szz, const PmlDev<C>& q, C l2)
 {
-    const C dv = (q.phx[i] - q.phx[i - 1]) + (q.phy[i] - q.phy[i - sy]) + (q.phz[i] - q.phz[i - sz]);
-    const C ps = q.psi[i];
+    const C dv = (q.phx[k] - (kxm >= 0 ? q.phx[kxm] : (C)0)) + (q.phy[k] - (kym >= 0 ? q.phy[kym] : (C)0)) +
+                 (q.phz[k] - (kzm >= 0 ? q.phz[kzm] : (C)0));
+    const C ps = q.psi[k];
     const C psn = ps + q.dt * pc;
-    q.psi[i] = psn;
+    q.psi[k] = psn;
     return pml_formula<C>(sum, pc, pm, dv, ps, psn, sx, syy, szz, q.dt, q.dx, l2);
+}
+
+// Compact index of cell (x, y, z) of room r in the PML phi/psi arrays, or -1
+// outside the room's PML region.  boxes: [nbox][8] = room, x0, x1, y0, y1,
+// z0, z1 (half-open), base; a box is stored x-fastest from base.  The boxes
+// of a room are disjoint, cover its PML region and sit at box0 .. box1 of the
+// table; `hint` (a box of the same room, usually the cell's own) is tried first.
+__device__ __forceinline__ int64_t pml_find(const int32_t* __restrict__ boxes, const RoomDev& r, int hint, int x,
+                                            int y, int z)
+{
+    for (int k = -1; k < r.box1 - r.box0; ++k) {
+        const int32_t* b = boxes + 8 * (k < 0 ? hint : r.box0 + k);
+        if (x >= b[1] && x < b[2] && y >= b[3] && y < b[4] && z >= b[5] && z < b[6])
+            return (int64_t)b[7] + ((int64_t)(z - b[5]) * (b[4] - b[3]) + (y - b[3])) * (b[2] - b[1]) + (x - b[1]);
+    }
+    return -1;
 }
 
 // Crank-Nicolson step of one phi component at a +1/2 point: `old` is phi^n,
@@ -96,15 +116,13 @@ __device__ __forceinline__ bool in_pml(const RoomDev& r, int x, int y, int z)
 // One thread owns an (x, y) column of a TX x TY x TZ tile and marches z,
 // carrying p^n at z-1, z, z+1 in registers; x and y neighbours come from the
 // same and adjacent 64-lane rows (L1 / L2 hits).  The sum order is the
-// reference's: x+1, x-1, y+1, y-1, z+1, z-1.  With PML, cells inside the
-// room's PML slabs take the PML update instead.  With MASK, cells whose bit
-// is set in `mask` (boundary and source/receiver cells, finished by other
-// threads of the same launch) are left alone, PML state included.
-template <typename S, typename C, int TZ, bool PML, bool MASK>
+// reference's: x+1, x-1, y+1, y-1, z+1, z-1.  With MASK, cells whose bit is
+// set in `mask` (boundary, source/receiver and PML cells, finished by other
+// threads of the same launch) are left alone.
+template <typename S, typename C, int TZ, bool MASK>
 __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __restrict__ cur,
                                                 const S* __restrict__ prev, const RoomDev& r, int x, int y,
-                                                int z0, C l2, C c2, const PmlDev<C>& q,
-                                                const uint32_t* __restrict__ mask)
+                                                int z0, C l2, C c2, const uint32_t* __restrict__ mask)
 {
     if (x < 1 || x >= r.nx - 1 || y < 1 || y >= r.ny - 1) return;
     const int zs = z0 < 1 ? 1 : z0;
@@ -112,13 +130,6 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
     if (zs >= ze) return;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + x;
-    C sx = 0, syy = 0;
-    bool col_pml = false;
-    if (PML) {
-        sx = q.z[0][r.axis_off[0] + x];
-        syy = q.z[1][r.axis_off[1] + y];
-        col_pml = x < r.pml_lo[0] || x >= r.pml_hi[0] || y < r.pml_lo[1] || y >= r.pml_hi[1];
-    }
     // Software pipeline with two register sets A and B used alternately: the
     // loads of plane z+1 are issued before plane z is computed and stored,
     // and no loaded register is copied while its load may be in flight.
@@ -136,7 +147,7 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
         P.mw = MASK ? mask[j >> 5] : 0u;
     };
     // (pd, pc) = p^n at z-1 and z of the plane being computed
-    auto do_plane = [&](const Plane& P, int64_t j, int z, C pd, C pc) {
+    auto do_plane = [&](const Plane& P, int64_t j, C pd, C pc) {
         C s = P.xp + P.xm;
         s = s + P.yp;
         s = s + P.ym;
@@ -144,12 +155,8 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
         s = s + pd;
         const bool skip = MASK && ((P.mw >> (j & 31)) & 1u);
         if (!skip) {
-            if (PML && (col_pml || z < r.pml_lo[2] || z >= r.pml_hi[2])) {
-                st<S, C>(out, j, pml_value<C>(s, pc, P.pm, j, sy, sz, sx, syy, q.z[2][r.axis_off[2] + z], q, l2));
-            } else {
-                const C v = l2 * s + c2 * pc;
-                st<S, C>(out, j, v - P.pm);
-            }
+            const C v = l2 * s + c2 * pc;
+            st<S, C>(out, j, v - P.pm);
         }
     };
     // Planes go in pairs (z, z1 = min(z+1, ze-1)); with an odd count the last
@@ -165,12 +172,12 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
         const int64_t i1 = odd_tail ? i : i + sz;
         const int64_t i2 = (z + 2 < ze) ? i + 2 * sz : i_last;
         load_plane(B, i1);
-        do_plane(A, i, z, pd, pc);
+        do_plane(A, i, pd, pc);
         const C au = A.u;  // p^n at z+1, already consumed above
         load_plane(A, i2);
         const C pd1 = odd_tail ? pd : pc;
         const C pc1 = odd_tail ? pc : au;
-        do_plane(B, i1, odd_tail ? z : z + 1, pd1, pc1);
+        do_plane(B, i1, pd1, pc1);
         pd = pc1;
         pc = B.u;
     }
@@ -188,8 +195,7 @@ __global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const
     const RoomDev r = rooms[t.x];
     const int x = t.y + (int)(threadIdx.x % TX);
     const int y = t.z + (int)(threadIdx.x / TX);
-    const PmlDev<C> none{};
-    interior_column<S, C, TZ, false, false>(out, cur, prev, r, x, y, t.w, l2, c2, none, nullptr);
+    interior_column<S, C, TZ, false>(out, cur, prev, r, x, y, t.w, l2, c2, nullptr);
 }
 
 // ---------------------------------------------------------------- boundary
@@ -345,10 +351,16 @@ struct BndCtx {
     C* psib[2];
     const C* zh[3];
     C cc, idt, idx;
-    const int32_t* pml_cell;
-    const uint32_t* pml_xyz;
-    const int32_t* pml_flag;  // bits 24-31 room; bits 0-23 kind + 2, kind -2: boundary cell (phi/psi
-                              // only), -1: plain, >= 0: special-cell index
+    // PML cell records {x | y << 11 | z << 22, word, k, nb}: word bits 24-31
+    // room, bits 0-23 kind + 2 (kind -2: boundary cell, phi/psi only; -1:
+    // plain; >= 0: special-cell index); k the cell's compact index; nb bits
+    // 0-10 / 11-21 the x / y extent of its box, bits 22-27 which of the
+    // neighbours +x, -x, +y, -y, +z, -z lie in the same box, bit 28 set when
+    // one of the others lies in another PML box: then pml_side[q] holds the
+    // box (bits 0-25) and those neighbours (bits 26-31), found by pml_find
+    const int4* pml_rec;
+    const uint32_t* pml_side;
+    const int32_t* pml_boxes;
 };
 
 __device__ __forceinline__ void unpack_xyz(uint32_t p, int& x, int& y, int& z)
@@ -436,16 +448,16 @@ __device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsi
 // split path runs after step n-1, moved into the next launch.  Each PML cell
 // stores its own phi^n and recomputes phi^n at its -x, -y, -z neighbours for
 // the divergence, so no cell waits on another.
-__device__ __forceinline__ bool in_region(const RoomDev& r, int x, int y, int z)
-{
-    return x >= 1 && x <= r.nx - 2 && y >= 1 && y <= r.ny - 2 && z >= 1 && z <= r.nz - 2 && in_pml(r, x, y, z);
-}
-
+// phi^n of component d at cell (x, y, z) = global index j of room r, from
+// phi^{n-1}, p^n, p^{n-1} and psi^{n-1/2}: kj is the cell's compact index
+// (-1: outside the PML region, where phi is zero), kn that of its +
+// neighbour along d (-1: psi is zero there).
 template <typename S, typename C>
 __device__ __forceinline__ C phi_at(const BndCtx<S, C>& c, const RoomDev& r, int d, int64_t j, int x, int y, int z,
-                                    const C* __restrict__ phr, const C* __restrict__ psr, int64_t st)
+                                    int64_t kj, int64_t kn, const C* __restrict__ phr, const C* __restrict__ psr,
+                                    int64_t st)
 {
-    if (!in_region(r, x, y, z)) return (C)0;
+    if (kj < 0) return (C)0;
     const C* zc0 = c.pml.z[0] + r.axis_off[0];
     const C* zc1 = c.pml.z[1] + r.axis_off[1];
     const C* zc2 = c.pml.z[2] + r.axis_off[2];
@@ -463,59 +475,113 @@ __device__ __forceinline__ C phi_at(const BndCtx<S, C>& c, const RoomDev& r, int
         g2 = zc0[x] + zc1[y] - h;
         g3 = zc0[x] * zc1[y];
     }
-    return phi_update<C>(phr[j], ld<S, C>(c.cur, j), ld<S, C>(c.cur, j + st), ld<S, C>(c.prev, j),
-                         ld<S, C>(c.prev, j + st), psr[j], psr[j + st], h, g2, g3, c.cc, c.idt, c.idx);
+    return phi_update<C>(phr[kj], ld<S, C>(c.cur, j), ld<S, C>(c.cur, j + st), ld<S, C>(c.prev, j),
+                         ld<S, C>(c.prev, j + st), psr[kj], kn >= 0 ? psr[kn] : (C)0, h, g2, g3, c.cc, c.idt,
+                         c.idx);
+}
+
+// One PML cell of the fused launch in two parts.  pml_first: phi^n at the
+// cell (stored), psi^{n+1/2} (stored).  pml_second: unless it is a boundary
+// cell, the PML pressure from phi^n at the -x, -y, -z neighbours, given
+// (have[d]) or recomputed.
+template <typename C> struct PmlCell {
+    RoomDev r;
+    int q, flag, x, y, z;
+    uint32_t nb;
+    int64_t i, k;
+    C fx, fy, fz, pc, ps, psn;
+};
+
+// Compact index of the neighbour of PML cell u in direction dir (0..5: +x,
+// -x, +y, -y, +z, -z), or -1 outside the PML region.
+template <typename S, typename C>
+__device__ __forceinline__ int64_t pml_nb(const BndCtx<S, C>& c, const PmlCell<C>& u, int dir)
+{
+    const int64_t wx = u.nb & 2047u, wxy = wx * ((u.nb >> 11) & 2047u);
+    const int64_t step = (dir >> 1) == 0 ? 1 : ((dir >> 1) == 1 ? wx : wxy);
+    if ((u.nb >> (22 + dir)) & 1u) return (dir & 1) ? u.k - step : u.k + step;
+    if (!((u.nb >> 28) & 1u)) return -1;
+    const uint32_t side = c.pml_side[u.q];
+    if (!((side >> (26 + dir)) & 1u)) return -1;
+    const int sgn = (dir & 1) ? -1 : 1;
+    return pml_find(c.pml_boxes, u.r, (int)(side & 0x3FFFFFFu), u.x + ((dir >> 1) == 0 ? sgn : 0),
+                    u.y + ((dir >> 1) == 1 ? sgn : 0), u.z + ((dir >> 1) == 2 ? sgn : 0));
 }
 
 template <typename S, typename C>
-__device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int k, int64_t t)
+__device__ __forceinline__ void pml_first(const BndCtx<S, C>& c, int q, int par, PmlCell<C>& u)
 {
-    const int par = (int)(t & 1);
-    const C* phr0 = c.phib[par ^ 1][0];
-    const C* phr1 = c.phib[par ^ 1][1];
-    const C* phr2 = c.phib[par ^ 1][2];
+    const int4 rec = c.pml_rec[q];
+    const uint32_t word = (uint32_t)rec.y;
+    u.q = q;
+    u.r = c.rooms[word >> 24];
+    u.flag = (int)(word & 0xFFFFFFu) - 2;
+    unpack_xyz((uint32_t)rec.x, u.x, u.y, u.z);
+    u.k = (uint32_t)rec.z;
+    u.nb = (uint32_t)rec.w;
+    const int64_t sy = u.r.nx, sz = (int64_t)u.r.nx * u.r.ny;
+    u.i = u.r.base + (int64_t)u.z * sz + (int64_t)u.y * sy + u.x;
     const C* psr = c.psib[par ^ 1];
-    const uint32_t word = (uint32_t)c.pml_flag[k];
-    const RoomDev r = c.rooms[word >> 24];
-    const int flag = (int)(word & 0xFFFFFFu) - 2;
-    const int64_t i = c.pml_cell[k];
-    int x, y, z;
-    unpack_xyz(c.pml_xyz[k], x, y, z);
-    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
-    const C fx = phi_at<S, C>(c, r, 0, i, x, y, z, phr0, psr, 1);
-    const C fy = phi_at<S, C>(c, r, 1, i, x, y, z, phr1, psr, sy);
-    const C fz = phi_at<S, C>(c, r, 2, i, x, y, z, phr2, psr, sz);
-    c.phib[par][0][i] = fx;
-    c.phib[par][1][i] = fy;
-    c.phib[par][2][i] = fz;
-    const C pc = ld<S, C>(c.cur, i);
-    const C ps = psr[i];
-    const C psn = ps + c.pml.dt * pc;
-    c.psib[par][i] = psn;
-    if (flag == -2) return;  // boundary cell: its pressure comes from the LRS update
-    const C fxm = phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, phr0, psr, 1);
-    const C fym = phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, phr1, psr, sy);
-    const C fzm = phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, phr2, psr, sz);
-    const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
-    C s = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
-    s = s + ld<S, C>(c.cur, i + sy);
-    s = s + ld<S, C>(c.cur, i - sy);
-    s = s + ld<S, C>(c.cur, i + sz);
-    s = s + ld<S, C>(c.cur, i - sz);
-    const C v = pml_formula<C>(s, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
-                               c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt, c.pml.dx,
-                               c.k.l2);
-    if (flag >= 0)
-        finish_special<S, C>(c, flag, i, t, rt<S, C>(v));
+    const int64_t kxp = pml_nb<S, C>(c, u, 0);
+    const int64_t kyp = pml_nb<S, C>(c, u, 2);
+    const int64_t kzp = pml_nb<S, C>(c, u, 4);
+    u.fx = phi_at<S, C>(c, u.r, 0, u.i, u.x, u.y, u.z, u.k, kxp, c.phib[par ^ 1][0], psr, 1);
+    u.fy = phi_at<S, C>(c, u.r, 1, u.i, u.x, u.y, u.z, u.k, kyp, c.phib[par ^ 1][1], psr, sy);
+    u.fz = phi_at<S, C>(c, u.r, 2, u.i, u.x, u.y, u.z, u.k, kzp, c.phib[par ^ 1][2], psr, sz);
+    c.phib[par][0][u.k] = u.fx;
+    c.phib[par][1][u.k] = u.fy;
+    c.phib[par][2][u.k] = u.fz;
+    u.pc = ld<S, C>(c.cur, u.i);
+    u.ps = psr[u.k];
+    u.psn = u.ps + c.pml.dt * u.pc;
+    c.psib[par][u.k] = u.psn;
+}
+
+template <typename S, typename C>
+__device__ __forceinline__ void pml_second(const BndCtx<S, C>& c, const PmlCell<C>& u, int par, int64_t t, C fxm,
+                                           C fym, C fzm, bool hx, bool hy, bool hz)
+{
+    const int64_t sy = u.r.nx, sz = (int64_t)u.r.nx * u.r.ny;
+    const C* psr = c.psib[par ^ 1];
+    if (!hx)
+        fxm = phi_at<S, C>(c, u.r, 0, u.i - 1, u.x - 1, u.y, u.z, pml_nb<S, C>(c, u, 1), u.k, c.phib[par ^ 1][0], psr, 1);
+    if (!hy)
+        fym = phi_at<S, C>(c, u.r, 1, u.i - sy, u.x, u.y - 1, u.z, pml_nb<S, C>(c, u, 3), u.k, c.phib[par ^ 1][1], psr,
+                           sy);
+    if (!hz)
+        fzm = phi_at<S, C>(c, u.r, 2, u.i - sz, u.x, u.y, u.z - 1, pml_nb<S, C>(c, u, 5), u.k, c.phib[par ^ 1][2], psr,
+                           sz);
+    const C dv = (u.fx - fxm) + (u.fy - fym) + (u.fz - fzm);
+    const int64_t i = u.i;
+    C sum = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
+    sum = sum + ld<S, C>(c.cur, i + sy);
+    sum = sum + ld<S, C>(c.cur, i - sy);
+    sum = sum + ld<S, C>(c.cur, i + sz);
+    sum = sum + ld<S, C>(c.cur, i - sz);
+    const C v = pml_formula<C>(sum, u.pc, ld<S, C>(c.prev, i), dv, u.ps, u.psn, c.pml.z[0][u.r.axis_off[0] + u.x],
+                               c.pml.z[1][u.r.axis_off[1] + u.y], c.pml.z[2][u.r.axis_off[2] + u.z], c.pml.dt,
+                               c.pml.dx, c.k.l2);
+    if (u.flag >= 0)
+        finish_special<S, C>(c, u.flag, i, t, rt<S, C>(v));
     else
         st<S, C>(c.out, i, v);
 }
 
+// PML cell q on its own: phi^n at the -x, -y, -z neighbours is recomputed.
+template <typename S, typename C>
+__device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int q, int64_t t)
+{
+    const int par = (int)(t & 1);
+    PmlCell<C> u;
+    pml_first<S, C>(c, q, par, u);
+    if (u.flag == -2) return;  // boundary cell: its pressure comes from the LRS update
+    pml_second<S, C>(c, u, par, t, (C)0, (C)0, (C)0, false, false, false);
+}
+
 // PML cells [a, b) of one work chunk, one per thread of the workgroup.  Each
-// thread computes phi^n at its own cell and shares it through LDS; the -x,
-// -y and -z neighbours' phi^n is read from there when the neighbour is in the
-// chunk (nbd[q]: list-index distances, 10 bits each, 0 = not in the chunk)
-// and recomputed otherwise.  Values are those of pml_entry.
+// thread shares its phi^n through LDS; a neighbour's phi^n is read from there
+// when the neighbour is in the chunk (nbd[q]: list-index distances, 10 bits
+// per axis, 0 = not in the chunk) and recomputed otherwise.
 template <typename S, typename C, int NT>
 __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, int64_t t,
                                           const uint32_t* __restrict__ nbd, C (*lds)[NT])
@@ -524,56 +590,20 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
     const int q = a + tid;
     const bool valid = q < b;
     const int par = (int)(t & 1);
-    const C* phr0 = c.phib[par ^ 1][0];
-    const C* phr1 = c.phib[par ^ 1][1];
-    const C* phr2 = c.phib[par ^ 1][2];
-    const C* psr = c.psib[par ^ 1];
-    RoomDev r{};
-    int flag = -2, x = 0, y = 0, z = 0;
-    int64_t i = 0;
-    C fx = 0, fy = 0, fz = 0, pc = 0, ps = 0, psn = 0;
+    PmlCell<C> u;
+    u.flag = -2;
     if (valid) {
-        const uint32_t word = (uint32_t)c.pml_flag[q];
-        r = c.rooms[word >> 24];
-        flag = (int)(word & 0xFFFFFFu) - 2;
-        i = c.pml_cell[q];
-        unpack_xyz(c.pml_xyz[q], x, y, z);
-        const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
-        fx = phi_at<S, C>(c, r, 0, i, x, y, z, phr0, psr, 1);
-        fy = phi_at<S, C>(c, r, 1, i, x, y, z, phr1, psr, sy);
-        fz = phi_at<S, C>(c, r, 2, i, x, y, z, phr2, psr, sz);
-        c.phib[par][0][i] = fx;
-        c.phib[par][1][i] = fy;
-        c.phib[par][2][i] = fz;
-        pc = ld<S, C>(c.cur, i);
-        ps = psr[i];
-        psn = ps + c.pml.dt * pc;
-        c.psib[par][i] = psn;
-        lds[0][tid] = fx;
-        lds[1][tid] = fy;
-        lds[2][tid] = fz;
+        pml_first<S, C>(c, q, par, u);
+        lds[0][tid] = u.fx;
+        lds[1][tid] = u.fy;
+        lds[2][tid] = u.fz;
     }
     __syncthreads();
-    if (!valid || flag == -2) return;  // boundary cells: pressure from the LRS update
-    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    if (!valid || u.flag == -2) return;
     const uint32_t nb = nbd[q];
     const int dxm = (int)(nb & 1023u), dym = (int)((nb >> 10) & 1023u), dzm = (int)(nb >> 20);
-    const C fxm = dxm ? lds[0][tid - dxm] : phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, phr0, psr, 1);
-    const C fym = dym ? lds[1][tid - dym] : phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, phr1, psr, sy);
-    const C fzm = dzm ? lds[2][tid - dzm] : phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, phr2, psr, sz);
-    const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
-    C sum = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
-    sum = sum + ld<S, C>(c.cur, i + sy);
-    sum = sum + ld<S, C>(c.cur, i - sy);
-    sum = sum + ld<S, C>(c.cur, i + sz);
-    sum = sum + ld<S, C>(c.cur, i - sz);
-    const C v = pml_formula<C>(sum, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
-                               c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt, c.pml.dx,
-                               c.k.l2);
-    if (flag >= 0)
-        finish_special<S, C>(c, flag, i, t, rt<S, C>(v));
-    else
-        st<S, C>(c.out, i, v);
+    pml_second<S, C>(c, u, par, t, dxm ? lds[0][tid - dxm] : (C)0, dym ? lds[1][tid - dym] : (C)0,
+                     dzm ? lds[2][tid - dzm] : (C)0, dxm != 0, dym != 0, dzm != 0);
 }
 
 template <typename S, typename C>
@@ -631,9 +661,9 @@ __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
     c.cc = cv * cv;
     c.idt = (C)1.0 / c.pml.dt;
     c.idx = (C)1.0 / c.pml.dx;
-    c.pml_cell = a.pml_cell;
-    c.pml_xyz = a.pml_xyz;
-    c.pml_flag = a.pml_flag;
+    c.pml_rec = a.pml_rec;
+    c.pml_side = a.pml_side;
+    c.pml_boxes = a.pml_boxes;
     return c;
 }
 
@@ -690,7 +720,7 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
             const RoomDev r = c.rooms[t4.x];
             const int x = t4.y + (int)(threadIdx.x % TX);
             const int y = t4.z + (int)(threadIdx.x / TX);
-            interior_column<S, C, TZ, false, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml, mask);
+            interior_column<S, C, TZ, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, mask);
             return;
         }
         if (wk.x < 0) return;
@@ -714,7 +744,7 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
         const RoomDev r = c.rooms[t4.x];
         const int x = t4.y + (int)(threadIdx.x % TX);
         const int y = t4.z + (int)(threadIdx.x / TX);
-        interior_column<S, C, TZ, false, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, c.pml, mask);
+        interior_column<S, C, TZ, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, mask);
         return;
     }
     const int64_t t = *d_t + t_rel;
@@ -732,6 +762,19 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
     }
 }
 
+// Plain copy, the calibration kernel of afx_copy_probe.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const float4*)src, (float4*)dst, n16);
+    return hipGetLastError();
+}
+
 __global__ void k_advance_t(int64_t* d_t, int64_t k)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *d_t += k;
@@ -746,7 +789,7 @@ __global__ void k_pml_p(PmlArgs a)
 {
     using C = typename Compute<S>::T;
     // grid: x*y flattened on blockIdx.x, z plane on blockIdx.y, box on blockIdx.z
-    const int32_t* bx = a.boxes + 7 * blockIdx.z;
+    const int32_t* bx = a.boxes + 8 * blockIdx.z;
     const int room = bx[0];
     const int wx = bx[2] - bx[1], wy = bx[4] - bx[3];
     const RoomDev r = a.rooms[room];
@@ -772,6 +815,11 @@ __global__ void k_pml_p(PmlArgs a)
         const int x = bx[1] + rr % wx;
         const int y = bx[3] + rr / wx;
         const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
+        const int64_t k = (int64_t)bx[7] + (int64_t)(z - bx[5]) * wx * wy + rr;
+        const int hint = (int)blockIdx.z;
+        const int64_t kxm = pml_find(a.boxes, r, hint, x - 1, y, z);
+        const int64_t kym = pml_find(a.boxes, r, hint, x, y - 1, z);
+        const int64_t kzm = pml_find(a.boxes, r, hint, x, y, z - 1);
         const C pc = ld<S, C>(cur, i);
         const C pm = ld<S, C>(prev, i);
         C sum = ld<S, C>(cur, i + 1) + ld<S, C>(cur, i - 1);
@@ -779,7 +827,7 @@ __global__ void k_pml_p(PmlArgs a)
         sum = sum + ld<S, C>(cur, i - sy);
         sum = sum + ld<S, C>(cur, i + sz);
         sum = sum + ld<S, C>(cur, i - sz);
-        st<S, C>(out, i, pml_value<C>(sum, pc, pm, i, sy, sz, zx[x], zy[y], zz[z], pq, l2));
+        st<S, C>(out, i, pml_value<C>(sum, pc, pm, k, kxm, kym, kzm, zx[x], zy[y], zz[z], pq, l2));
     }
 }
 
@@ -789,7 +837,7 @@ __global__ void k_pml_phi(PmlArgs a)
 {
     using C = typename Compute<S>::T;
     // grid: x*y flattened on blockIdx.x, z plane on blockIdx.y, box on blockIdx.z
-    const int32_t* bx = a.boxes + 7 * blockIdx.z;
+    const int32_t* bx = a.boxes + 8 * blockIdx.z;
     const int room = bx[0];
     const int wx = bx[2] - bx[1], wy = bx[4] - bx[3];
     const RoomDev r = a.rooms[room];
@@ -815,21 +863,26 @@ __global__ void k_pml_phi(PmlArgs a)
         const int x = bx[1] + rr % wx;
         const int y = bx[3] + rr / wx;
         const int64_t i = r.base + (int64_t)z * sz + (int64_t)y * sy + x;
+        const int64_t k = (int64_t)bx[7] + (int64_t)(z - bx[5]) * wx * wy + rr;
+        const int hint = (int)blockIdx.z;
+        const int64_t kxp = pml_find(a.boxes, r, hint, x + 1, y, z);
+        const int64_t kyp = pml_find(a.boxes, r, hint, x, y + 1, z);
+        const int64_t kzp = pml_find(a.boxes, r, hint, x, y, z + 1);
         const C p1c = ld<S, C>(p1, i), pcc = ld<S, C>(p, i);
         {
             const C h = hx[x];
-            phx[i] = phi_update<C>(phx[i], p1c, ld<S, C>(p1, i + 1), pcc, ld<S, C>(p, i + 1), psi[i], psi[i + 1], h,
-                                   zy[y] + zz[z] - h, zy[y] * zz[z], cc, idt, idx);
+            phx[k] = phi_update<C>(phx[k], p1c, ld<S, C>(p1, i + 1), pcc, ld<S, C>(p, i + 1), psi[k],
+                                   kxp >= 0 ? psi[kxp] : (C)0, h, zy[y] + zz[z] - h, zy[y] * zz[z], cc, idt, idx);
         }
         {
             const C h = hy[y];
-            phy[i] = phi_update<C>(phy[i], p1c, ld<S, C>(p1, i + sy), pcc, ld<S, C>(p, i + sy), psi[i], psi[i + sy],
-                                   h, zx[x] + zz[z] - h, zx[x] * zz[z], cc, idt, idx);
+            phy[k] = phi_update<C>(phy[k], p1c, ld<S, C>(p1, i + sy), pcc, ld<S, C>(p, i + sy), psi[k],
+                                   kyp >= 0 ? psi[kyp] : (C)0, h, zx[x] + zz[z] - h, zx[x] * zz[z], cc, idt, idx);
         }
         {
             const C h = hz[z];
-            phz[i] = phi_update<C>(phz[i], p1c, ld<S, C>(p1, i + sz), pcc, ld<S, C>(p, i + sz), psi[i], psi[i + sz],
-                                   h, zx[x] + zy[y] - h, zx[x] * zy[y], cc, idt, idx);
+            phz[k] = phi_update<C>(phz[k], p1c, ld<S, C>(p1, i + sz), pcc, ld<S, C>(p, i + sz), psi[k],
+                                   kzp >= 0 ? psi[kzp] : (C)0, h, zx[x] + zy[y] - h, zx[x] * zy[y], cc, idt, idx);
         }
     }
 }

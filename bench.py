@@ -145,6 +145,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=4096)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (HBM counter calibration) and print its rate")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-steps", type=int, default=30000)
     args = ap.parse_args()
@@ -157,6 +159,11 @@ def main():
     # share a process with the one libafx.so links.  The timed region is
     # bracketed by device-wide synchronisation through the library instead.
     from anc_field_py import _afx, rooms
+
+    if args.copy_probe:
+        print(json.dumps({"copy_bytes": args.copy_probe, "reps": 3,
+                          "gbps": round(_afx.copy_probe(args.copy_probe, reps=3), 1)}), flush=True)
+        return
 
     total = args.warmup + args.steps + args.profile_steps
     w = rooms.furnished(steps=total)
@@ -180,10 +187,16 @@ def main():
     bpc = BYTES_PER_CELL[args.precision]
     achieved = bpc * w.cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     kernel = "k_fused" if st.fused else "k_interior"
+    # a plain device copy moving the same bytes (read half, write half): the
+    # practical rate for this working-set size
+    copy_bytes = (bpc * w.cells // 2) // 16 * 16
+    copy_gbps = _afx.copy_probe(copy_bytes, reps=50)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(f"{w.name}_{args.precision}"),
             "kernel": kernel, "kernel_ms": round(k_ms, 5),
-            "algorithmic_bytes_per_launch": bpc * w.cells}
+            "algorithmic_bytes_per_launch": bpc * w.cells,
+            "plain_copy_same_bytes": {"gbps": round(copy_gbps, 1),
+                                      "us": round(2 * copy_bytes / (copy_gbps * 1e9) * 1e6, 2)}}
     out = {
         "metric": "Gcell-updates/s",
         "value": round(gcells, 2),

@@ -182,6 +182,13 @@ int afx_set_tile(afx_sim* sim, int32_t tx, int32_t ty, int32_t tz);
  * unchanged. */
 int afx_autotune(afx_sim* sim, int32_t steps_per_trial, int32_t tile[3]);
 
+/* Measurement aid, no counterpart in the reference: a plain device copy of
+ * `bytes` (read `bytes`, write `bytes`, float4 grid-stride kernel k_copy)
+ * repeated `reps` times after one warm-up; *gbps = 2*bytes*reps / device time.
+ * It is the calibration kernel for HBM counters and the practical copy rate
+ * next to the 8 TB/s peak. */
+int afx_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif
