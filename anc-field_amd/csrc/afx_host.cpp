@@ -292,9 +292,13 @@ hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t st)
     return e;
 }
 
-// Tile table in XCD-swizzled order: workgroup i runs on XCD i % 8, so tile
-// slot i takes logical tile (i % 8) * per + i / 8 and each XCD walks a
-// contiguous run of tiles whose halos it shares in its own L2.
+// Tile table in XCD-swizzled order.  Workgroup i runs on XCD i % 8.
+// Logical tiles (x fastest, then y, then z) are dealt to the XCDs in groups
+// of G consecutive tiles, G an eighth of room 0's z-layer of tiles, round
+// robin with the starting XCD advanced by one per layer: every XCD gets a
+// share of each layer (the z walls) and of each band of y (the y walls and
+// slabs), and each group is a run of y-adjacent tiles whose halos stay in
+// that XCD's L2.  Slot i holds the (i / 8)-th tile of XCD i % 8.
 void build_tiles(afx_sim* s, std::vector<int4>& table, std::vector<int32_t>& logical_of_slot)
 {
     const TileShape ts = kTiles[s->tile];
@@ -306,16 +310,21 @@ void build_tiles(afx_sim* s, std::vector<int4>& table, std::vector<int32_t>& log
                 for (int x0 = 0; x0 < r.nx; x0 += ts.tx) logical.push_back(make_int4((int)ri, x0, y0, z0));
     }
     const int64_t n = (int64_t)logical.size();
-    const int64_t per = (n + 7) / 8;
+    const Room& r0 = s->rooms[0];
+    const int64_t layer = (int64_t)((r0.nx + ts.tx - 1) / ts.tx) * ((r0.ny + ts.ty - 1) / ts.ty);
+    const int64_t G = std::max<int64_t>(1, layer / 8);
+    std::vector<std::vector<int64_t>> of_xcd(8);
+    for (int64_t L = 0; L < n; ++L) of_xcd[(size_t)((L / G + L / std::max<int64_t>(layer, 1)) % 8)].push_back(L);
+    int64_t per = 0;
+    for (const auto& v : of_xcd) per = std::max<int64_t>(per, (int64_t)v.size());
     table.assign((size_t)(per * 8), make_int4(-1, 0, 0, 0));
     logical_of_slot.assign((size_t)(per * 8), -1);
-    for (int64_t i = 0; i < per * 8; ++i) {
-        const int64_t L = (i % 8) * per + i / 8;
-        if (L < n) {
+    for (int64_t x = 0; x < 8; ++x)
+        for (size_t k = 0; k < of_xcd[(size_t)x].size(); ++k) {
+            const int64_t i = (int64_t)k * 8 + x, L = of_xcd[(size_t)x][k];
             table[(size_t)i] = logical[(size_t)L];
             logical_of_slot[(size_t)i] = (int32_t)L;
         }
-    }
 }
 
 int build_tile_tables(afx_sim* s)
