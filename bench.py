@@ -6,7 +6,7 @@ face, band-limited noise source, 16 receivers), FP32, physical boundary
 mode, run through the C ABI.  Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
-                  [--compat] [--no-cpu-baseline] [--tune] [--no-batched] [--no-surrogate]
+                  [--compat] [--no-cpu-baseline] [--tune] [--no-batched] [--no-surrogate] [--no-large]
 """
 import argparse
 import json
@@ -123,6 +123,35 @@ def surrogate_leg(train_steps):
         sur.close()
 
 
+def large_leg(steps):
+    """Large room, reduced-precision storage: 640 x 480 x 320 cells (98 M),
+    dx 0.01 m, 6 IIR materials, 16-cell PML on x+ and y+, band-limited
+    noise, 32 receivers.  The same run in FP64, FP32, FP16 and BF16 storage;
+    receiver-trace error of each against FP64."""
+    import numpy as np
+    from anc_field_py import _afx, rooms
+    w = rooms.large(steps=steps)
+    res = {"cells": w.cells, "steps": steps, "receivers": int(len(w.mic_cells))}
+    traces = {}
+    for prec in ("fp64", "fp32", "fp16", "bf16"):
+        s = rooms.make_sim(w, _afx.Sim, precision=prec)
+        try:
+            s.run(1)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.run(steps - 1)
+            s.synchronize()
+            dt = time.perf_counter() - t0
+            traces[prec] = s.receivers().astype(np.float64)
+        finally:
+            s.close()
+        res[prec] = {"gcell_per_s": round(w.cells * (steps - 1) / dt / 1e9, 1), "wall_s": round(dt, 2)}
+    ref = traces["fp64"]
+    for prec in ("fp32", "fp16", "bf16"):
+        res[prec]["rel_l2_vs_fp64"] = float(np.linalg.norm(traces[prec] - ref) / np.linalg.norm(ref))
+    return res
+
+
 def traffic_for(name):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, if one exists for this workload."""
@@ -148,6 +177,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and print its rate")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=20000)
     ap.add_argument("--surrogate-steps", type=int, default=30000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -225,6 +256,8 @@ def main():
         out["batched"] = batched_leg(args.batched_steps)
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(args.surrogate_steps)
+    if not args.no_large:
+        out["large"] = large_leg(args.large_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
