@@ -334,7 +334,11 @@ class MlpConfig(C.Structure):
         ("out_dim", C.c_int32), ("batch", C.c_int32), ("device", C.c_int32), ("seed", C.c_uint32),
         ("freq_min", C.c_float), ("freq_max", C.c_float),
         ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("weight_decay", C.c_float),
+        ("flags", C.c_uint32),
     ]
+
+
+MLP_NO_GRAPH = 1
 
 
 class MlpStats(C.Structure):

@@ -5,8 +5,11 @@
 // positions in one room to the band-limited impulse response that
 // anc_field_py/ancutil.py:49-79 otherwise obtains from a full FDTD run.
 //
-// Kernels (gfx950, one 64-lane wave per workgroup, 32x32 output tile per
-// wave = 2x2 v_mfma_f32_16x16x32_bf16 blocks, FP32 accumulators):
+// Kernels (gfx950).  A GEMM workgroup is 4 waves on one 32x32 output tile:
+// each wave takes every 4th 32-wide slice of K on 2x2
+// v_mfma_f32_16x16x32_bf16 blocks with FP32 accumulators, and the four
+// partial tiles are summed through LDS before the epilogue (the batch-256
+// GEMMs have too few tiles to fill the chip one wave per tile):
 //   k_features       Fourier features of the positions -> X [rows][F] and X^T
 //   k_gemm<EPI>      C = A . B^T over K (A [M][K], B [N][K], both K-contiguous)
 //                    with a fused epilogue:
@@ -17,8 +20,10 @@
 //   k_gemm_grouped   all weight gradients of a step in one launch: dW_l = dY_l^T . X_l
 //                    (A = dY_l^T [out][B], B = X_l^T [in][B], contraction over the batch)
 //   k_colsum         bias gradients, db_l[n] = sum_b dY_l[b][n], all layers in one launch
-//   k_adam           AdamW on the FP32 master weights; refreshes the BF16 copies
-//                    W [out][in] (forward operand) and W^T [in][out] (backward operand)
+//   k_adam           AdamW on the FP32 master weights, 32x32 tiles of W per
+//                    workgroup; refreshes the BF16 copies W [out][in] (forward
+//                    operand) and W^T [in][out] (backward operand, written
+//                    through an LDS transpose)
 //
 // Every operand of an MFMA is read K-contiguous: the forward uses X and W,
 // the input gradient dY and W^T, the weight gradient dY^T and X^T.  The
@@ -57,6 +62,20 @@ constexpr int kMaxLayers = 16;
 
 enum Epi { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_LOSS = 2, EPI_BACK = 3 };
 
+// Training-step position kept on the device, so that a captured graph of
+// steps replays unchanged: the minibatch rows, the loss partials and the
+// learning-rate and bias-correction factors of step *step are found from
+// it.  With step == null every offset is zero and the arguments are used as
+// given.
+struct StepRef {
+    const int64_t* step;
+    int64_t rows_stride;   // minibatch rows advance per step
+    int64_t part_stride;   // loss partials advance per step
+    const float* sched;    // [step][3]: lr, 1 - beta1^t, 1 - beta2^t
+};
+
+__device__ inline int64_t step_of(const StepRef& r) { return r.step ? *r.step : 0; }
+
 struct Gemm {
     const bf16* A;       // [M][lda], K-contiguous
     const bf16* B;       // [N][ldb], K-contiguous
@@ -74,6 +93,7 @@ struct Gemm {
     const int* rows;
     const float* wrow;   // LOSS: per-row weight
     float* partial;      // LOSS: one loss partial per workgroup
+    StepRef sr;          // LOSS: rows and partial advance with the device step
 };
 
 struct GradProblem {
@@ -102,6 +122,8 @@ struct AdamTable {
     int rows[kMaxLayers], cols[kMaxLayers];
     bf16* wb[kMaxLayers];         // [rows][cols]
     bf16* wt[kMaxLayers];         // [cols][rows]
+    int tstart[kMaxLayers + 1];   // first workgroup of layer l's 32x32 tiles of W
+    int bofs[kMaxLayers + 1];     // first bias of layer l in the concatenated biases
     int n;
 };
 
@@ -121,7 +143,8 @@ __device__ inline float silu_grad(float z)
 __device__ inline void mma_block(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb, int K,
                                  int m0, int n0, f32x4 (&c)[2][2])
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = (lane >> 4) << 3;
     const bf16* pa0 = A + (size_t)(m0 + r) * lda + kq;
     const bf16* pa1 = pa0 + (size_t)16 * lda;
@@ -130,7 +153,7 @@ __device__ inline void mma_block(const bf16* __restrict__ A, int lda, const bf16
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) c[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 4
-    for (int k = 0; k < K; k += 32) {
+    for (int k = 32 * wave; k < K; k += 128) {
         const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(pa0 + k);
         const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(pa1 + k);
         const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(pb0 + k);
@@ -142,58 +165,78 @@ __device__ inline void mma_block(const bf16* __restrict__ A, int lda, const bf16
     }
 }
 
-template <int E>
-__global__ __launch_bounds__(64) void k_gemm(Gemm g)
+// Sums the four waves' partial 32x32 tiles through LDS.  Afterwards thread t
+// holds, for block (i, j) = (q >> 1, q & 1) with q = t / 64 and lane
+// l = t % 64, the four elements e < 4 at row 16 i + 4 (l >> 4) + e, column
+// 16 j + (l & 15) of the tile (the layout of one wave's accumulators).
+__device__ inline f32x4 reduce_waves(const f32x4 (&c)[2][2], float (*lds)[4][4][64])
 {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q < 4; ++q)
+        for (int e = 0; e < 4; ++e) lds[wave][q][e][lane] = c[q >> 1][q & 1][e];
+    __syncthreads();
+    const int q = wave;
+    f32x4 r;
+    for (int e = 0; e < 4; ++e) r[e] = ((lds[0][q][e][lane] + lds[1][q][e][lane]) + lds[2][q][e][lane]) + lds[3][q][e][lane];
+    return r;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_gemm(Gemm g)
+{
+    __shared__ float lds[4][4][4][64];
+    __shared__ float lpart[4];
     const int tiles_n = g.N >> 5;
     const int m0 = (int)(blockIdx.x / tiles_n) << 5;
     const int n0 = (int)(blockIdx.x % tiles_n) << 5;
     f32x4 c[2][2];
     mma_block(g.A, g.lda, g.B, g.ldb, g.K, m0, n0, c);
+    const f32x4 acc = reduce_waves(c, lds);
 
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = q >> 1, j = q & 1;
+    const int n = n0 + 16 * j + (lane & 15);
+    const int mb = m0 + 16 * i + ((lane >> 4) << 2);
     float loss = 0.0f;
-    for (int i = 0; i < 2; ++i) {
-        for (int j = 0; j < 2; ++j) {
-            const int n = n0 + 16 * j + (lane & 15);
-            const int mb = m0 + 16 * i + ((lane >> 4) << 2);
-            bf16x4 col;
-            for (int e = 0; e < 4; ++e) {
-                const int m = mb + e;
-                float v = c[i][j][e];
-                if (E == EPI_HIDDEN) {
-                    const float z = v + g.bias[n];
-                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
-                    v = silu(z);
-                } else if (E == EPI_OUT) {
-                    g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
-                    continue;
-                } else if (E == EPI_LOSS) {
-                    const float y = v + g.bias[n];
-                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = y;
-                    const int src = g.rows ? g.rows[m] : m;
-                    const float w = g.wrow[src];
-                    const float d = y - g.target[(size_t)src * g.ldt + n];
-                    loss += w * (d * d);
-                    v = 2.0f * w * d;
-                } else {  // EPI_BACK
-                    v = v * silu_grad(g.zprev[(size_t)m * g.ldz + n]);
-                }
-                const bf16 h = (bf16)v;
-                g.act[(size_t)m * g.ldact + n] = h;
-                col[e] = h;
-            }
-            if (E != EPI_OUT && g.actT) *reinterpret_cast<bf16x4*>(g.actT + (size_t)n * g.ldactT + mb) = col;
+    bf16x4 col;
+    for (int e = 0; e < 4; ++e) {
+        const int m = mb + e;
+        float v = acc[e];
+        if (E == EPI_HIDDEN) {
+            const float z = v + g.bias[n];
+            if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
+            v = silu(z);
+        } else if (E == EPI_OUT) {
+            g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+            continue;
+        } else if (E == EPI_LOSS) {
+            const float y = v + g.bias[n];
+            if (g.f32) g.f32[(size_t)m * g.ldf + n] = y;
+            const int src = g.rows ? g.rows[step_of(g.sr) * g.sr.rows_stride + m] : m;
+            const float w = g.wrow[src];
+            const float d = y - g.target[(size_t)src * g.ldt + n];
+            loss += w * (d * d);
+            v = 2.0f * w * d;
+        } else {  // EPI_BACK
+            v = v * silu_grad(g.zprev[(size_t)m * g.ldz + n]);
         }
+        const bf16 h = (bf16)v;
+        g.act[(size_t)m * g.ldact + n] = h;
+        col[e] = h;
     }
+    if (E != EPI_OUT && g.actT) *reinterpret_cast<bf16x4*>(g.actT + (size_t)n * g.ldactT + mb) = col;
     if (E == EPI_LOSS) {
-        for (int s = 32; s > 0; s >>= 1) loss += __shfl_xor(loss, s);
-        if (lane == 0) g.partial[blockIdx.x] = loss;
+        for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
+        if (lane == 0) lpart[q] = loss;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            g.partial[step_of(g.sr) * g.sr.part_stride + blockIdx.x] = ((lpart[0] + lpart[1]) + lpart[2]) + lpart[3];
     }
 }
 
-__global__ __launch_bounds__(64) void k_gemm_grouped(GradTable t)
+__global__ __launch_bounds__(256) void k_gemm_grouped(GradTable t)
 {
+    __shared__ float lds[4][4][4][64];
     int p = 0;
     while (p + 1 < t.n && (int)blockIdx.x >= t.start[p + 1]) ++p;
     const GradProblem& g = t.p[p];
@@ -202,27 +245,25 @@ __global__ __launch_bounds__(64) void k_gemm_grouped(GradTable t)
     const int m0 = (tile / tiles_n) << 5, n0 = (tile % tiles_n) << 5;
     f32x4 c[2][2];
     mma_block(g.A, g.lda, g.B, g.ldb, g.K, m0, n0, c);
-    const int lane = threadIdx.x;
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            for (int e = 0; e < 4; ++e) {
-                const int m = m0 + 16 * i + ((lane >> 4) << 2) + e;
-                const int n = n0 + 16 * j + (lane & 15);
-                g.out[(size_t)m * g.N + n] = c[i][j][e];
-            }
+    const f32x4 acc = reduce_waves(c, lds);
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int n = n0 + 16 * (q & 1) + (lane & 15);
+    const int mb = m0 + 16 * (q >> 1) + ((lane >> 4) << 2);
+    for (int e = 0; e < 4; ++e) g.out[(size_t)(mb + e) * g.N + n] = acc[e];
 }
 
 // Fourier features: thread (b, j, k) writes sin and cos of 2 pi f_k x[b][j]
 // at features j*2F + k and j*2F + F + k, in X [rows][nfeat] and X^T [nfeat][ldt].
 __global__ void k_features(const float* __restrict__ x, const int* __restrict__ rows, int nrows, int in_dim, int nf,
-                           const float* __restrict__ freq, bf16* __restrict__ X, bf16* __restrict__ XT, int ldt)
+                           const float* __restrict__ freq, bf16* __restrict__ X, bf16* __restrict__ XT, int ldt,
+                           StepRef sr)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows * in_dim * nf) return;
     const int k = i % nf;
     const int j = (i / nf) % in_dim;
     const int b = i / (nf * in_dim);
-    const int src = rows ? rows[b] : b;
+    const int src = rows ? rows[step_of(sr) * sr.rows_stride + b] : b;
     const float cyc = freq[k] * x[(size_t)src * in_dim + j];
     float s, c;
     sincospif(2.0f * cyc, &s, &c);
@@ -252,15 +293,11 @@ __global__ __launch_bounds__(256) void k_colsum(ColsumTable t)
     if (lane == 0) t.out[l][n] = s;
 }
 
-// AdamW over all parameters (lr == 0 with clear == 0 only refreshes the BF16 copies).
-__global__ void k_adam(AdamTable t, float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2,
-                       const float* __restrict__ grad, float lr, float b1, float b2, float eps, float wd, float bc1,
-                       float bc2, int update)
+// AdamW on parameter i; returns its new value (the old one when !update).
+__device__ inline float adam_param(float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2,
+                                   const float* __restrict__ grad, int64_t i, float lr, float b1, float b2, float eps,
+                                   float wd, float bc1, float bc2, int update)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= t.off[t.n]) return;
-    int l = 0;
-    while (l + 1 < t.n && i >= t.off[l + 1]) ++l;
     float v = w[i];
     if (update) {
         const float gi = grad[i];
@@ -272,14 +309,58 @@ __global__ void k_adam(AdamTable t, float* __restrict__ w, float* __restrict__ m
         v = v - lr * (mh / (sqrtf(vh) + eps) + wd * v);
         w[i] = v;
     }
-    const int64_t local = i - t.off[l];
-    const int64_t nw = (int64_t)t.rows[l] * t.cols[l];
-    if (local < nw) {
-        const int r = (int)(local / t.cols[l]), c = (int)(local % t.cols[l]);
-        const bf16 h = (bf16)v;
-        t.wb[l][local] = h;
-        t.wt[l][(size_t)c * t.rows[l] + r] = h;
+    return v;
+}
+
+// AdamW over all parameters (update == 0 only refreshes the BF16 copies).
+// Workgroups [0, tstart[n]) take 32x32 tiles of the weight matrices: each
+// thread updates 4 consecutive weights of a row, writes their BF16 copies to
+// W and, through an LDS transpose, 4 consecutive elements of a row of W^T.
+// The remaining workgroups take the biases, 256 each.
+__global__ __launch_bounds__(256) void k_adam(AdamTable t, float* __restrict__ w, float* __restrict__ m1,
+                                              float* __restrict__ m2, const float* __restrict__ grad, float lr,
+                                              float b1, float b2, float eps, float wd, float bc1, float bc2, int update,
+                                              StepRef sr)
+{
+    __shared__ bf16 tr[32][34];
+    if (sr.sched) {
+        const float* q = sr.sched + 3 * step_of(sr);
+        lr = q[0];
+        bc1 = q[1];
+        bc2 = q[2];
     }
+    const int bid = (int)blockIdx.x;
+    if (bid >= t.tstart[t.n]) {
+        const int k = (bid - t.tstart[t.n]) * 256 + (int)threadIdx.x;
+        if (k >= t.bofs[t.n]) return;
+        int l = 0;
+        while (l + 1 < t.n && k >= t.bofs[l + 1]) ++l;
+        const int64_t i = t.off[l] + (int64_t)t.rows[l] * t.cols[l] + (k - t.bofs[l]);
+        adam_param(w, m1, m2, grad, i, lr, b1, b2, eps, wd, bc1, bc2, update);
+        return;
+    }
+    int l = 0;
+    while (l + 1 < t.n && bid >= t.tstart[l + 1]) ++l;
+    const int tile = bid - t.tstart[l];
+    const int cols = t.cols[l], rows = t.rows[l];
+    const int r0 = (tile / (cols >> 5)) << 5, c0 = (tile % (cols >> 5)) << 5;
+    const int rr = (int)threadIdx.x >> 3, cc = ((int)threadIdx.x & 7) << 2;
+    bf16x4 h;
+    for (int e = 0; e < 4; ++e) {
+        const int64_t i = t.off[l] + (int64_t)(r0 + rr) * cols + c0 + cc + e;
+        h[e] = (bf16)adam_param(w, m1, m2, grad, i, lr, b1, b2, eps, wd, bc1, bc2, update);
+        tr[cc + e][rr] = h[e];
+    }
+    *reinterpret_cast<bf16x4*>(t.wb[l] + (size_t)(r0 + rr) * cols + c0 + cc) = h;
+    __syncthreads();
+    bf16x4 o;
+    for (int e = 0; e < 4; ++e) o[e] = tr[rr][cc + e];
+    *reinterpret_cast<bf16x4*>(t.wt[l] + (size_t)(c0 + rr) * rows + r0 + cc) = o;
+}
+
+__global__ void k_step_inc(int64_t* step)
+{
+    if (threadIdx.x == 0) *step += 1;
 }
 
 #define HIP_CHECK(expr)                                                                                  \
@@ -338,6 +419,7 @@ struct afx_mlp {
     float* Y = nullptr;             // [B][out]
     float *xq = nullptr, *tq = nullptr, *wq = nullptr;  // staging for forward/gradients
     float* partial = nullptr;       // loss partials of one step
+    StepRef sr{};                   // device step position while training (null otherwise)
     int nblk_loss = 0;
     // training set
     float *xd = nullptr, *td = nullptr, *wd = nullptr;
@@ -352,7 +434,7 @@ namespace {
 int gemm_launch(afx_mlp* s, int epi, Gemm g)
 {
     if (g.M % 32 || g.N % 32 || g.K % 32) return err(AFX_E_ARG, "gemm %dx%dx%d not a multiple of 32", g.M, g.N, g.K);
-    const dim3 grid((unsigned)((g.M / 32) * (g.N / 32))), block(64);
+    const dim3 grid((unsigned)((g.M / 32) * (g.N / 32))), block(256);
     switch (epi) {
     case EPI_HIDDEN: k_gemm<EPI_HIDDEN><<<grid, block, 0, s->stream>>>(g); break;
     case EPI_OUT: k_gemm<EPI_OUT><<<grid, block, 0, s->stream>>>(g); break;
@@ -370,7 +452,7 @@ int forward_hidden(afx_mlp* s, const float* x, const int* rows, bool keep)
 {
     const int nthreads = s->B * s->cfg.in_dim * s->cfg.n_freq;
     k_features<<<(nthreads + 255) / 256, 256, 0, s->stream>>>(x, rows, s->B, s->cfg.in_dim, s->cfg.n_freq, s->freq,
-                                                             s->X, keep ? s->XT : nullptr, s->B);
+                                                             s->X, keep ? s->XT : nullptr, s->B, s->sr);
     HIP_CHECK(hipGetLastError());
     for (int l = 0; l + 1 < s->L; ++l) {
         Gemm g{};
@@ -421,6 +503,7 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
         g.rows = rows;
         g.wrow = weight;
         g.partial = s->partial;
+        g.sr = s->sr;
         if ((rc = gemm_launch(s, EPI_LOSS, g))) return rc;
     }
     for (int l = L - 1; l >= 1; --l) {  // input gradient of layer l = output gradient of layer l-1
@@ -455,7 +538,7 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
         p.K = s->B;
         gt.start[l + 1] = gt.start[l] + (p.M / 32) * (p.N / 32);
     }
-    k_gemm_grouped<<<gt.start[L], 64, 0, s->stream>>>(gt);
+    k_gemm_grouped<<<gt.start[L], 256, 0, s->stream>>>(gt);
     HIP_CHECK(hipGetLastError());
     ColsumTable ct{};
     ct.n = L;
@@ -478,11 +561,15 @@ int adam(afx_mlp* s, float lr, bool update)
     AdamTable at{};
     at.n = s->L;
     for (int l = 0; l <= s->L; ++l) at.off[l] = s->off[l];
+    at.tstart[0] = 0;
+    at.bofs[0] = 0;
     for (int l = 0; l < s->L; ++l) {
         at.rows[l] = s->out[l];
         at.cols[l] = s->in[l];
         at.wb[l] = s->wb[l];
         at.wt[l] = s->wt[l];
+        at.tstart[l + 1] = at.tstart[l] + (s->out[l] / 32) * (s->in[l] / 32);
+        at.bofs[l + 1] = at.bofs[l] + s->out[l];
     }
     float bc1 = 1.0f, bc2 = 1.0f;
     if (update) {
@@ -490,9 +577,9 @@ int adam(afx_mlp* s, float lr, bool update)
         bc1 = (float)(1.0 - std::pow((double)s->cfg.beta1, (double)s->t));
         bc2 = (float)(1.0 - std::pow((double)s->cfg.beta2, (double)s->t));
     }
-    const int64_t blocks = (s->nparam + 255) / 256;
+    const int64_t blocks = at.tstart[s->L] + (at.bofs[s->L] + 255) / 256;
     k_adam<<<(unsigned)blocks, 256, 0, s->stream>>>(at, s->w, s->m1, s->m2, s->g, lr, s->cfg.beta1, s->cfg.beta2,
-                                                    s->cfg.eps, s->cfg.weight_decay, bc1, bc2, update ? 1 : 0);
+                                                    s->cfg.eps, s->cfg.weight_decay, bc1, bc2, update ? 1 : 0, s->sr);
     HIP_CHECK(hipGetLastError());
     return AFX_OK;
 }
@@ -539,6 +626,7 @@ int afx_mlp_config_default(afx_mlp_config* c)
     c->beta2 = 0.999f;
     c->eps = 1e-8f;
     c->weight_decay = 0.0f;
+    c->flags = 0;
     return AFX_OK;
 }
 
@@ -820,43 +908,82 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
         }
         idx[i] = perm[pos++];
     }
-    int* d_idx = nullptr;
-    float* d_part = nullptr;
-    if (hipMalloc(&d_idx, idx.size() * sizeof(int)) != hipSuccess)
-        return err(AFX_E_NOMEM, "device allocation for the index stream failed");
-    if (hipMalloc(&d_part, (size_t)steps * s->nblk_loss * sizeof(float)) != hipSuccess) {
-        (void)hipFree(d_idx);
-        return err(AFX_E_NOMEM, "device allocation for the loss record failed");
-    }
-    auto cleanup = [&](int rc) {
-        (void)hipStreamSynchronize(s->stream);
-        (void)hipFree(d_idx);
-        (void)hipFree(d_part);
-        return rc;
-    };
-    if (hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, s->stream) != hipSuccess)
-        return cleanup(err(AFX_E_HIP, "index upload failed"));
-    const auto t0 = std::chrono::steady_clock::now();
-    if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
-    float* keep_partial = s->partial;
+    // per step: cosine learning rate from lr0 to lr1 and Adam's bias corrections
+    std::vector<float> sched((size_t)steps * 3);
+    const int64_t t0 = s->t;
     for (int64_t k = 0; k < steps; ++k) {
         const double frac = steps > 1 ? (double)k / (double)(steps - 1) : 1.0;
-        const float lr = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(M_PI * frac)));
-        s->partial = d_part + k * s->nblk_loss;
-        int rc = step_grad(s, s->xd, s->td, s->wd, d_idx + k * B, nullptr);
-        if (!rc) rc = adam(s, lr, true);
-        if (rc) {
-            s->partial = keep_partial;
-            return cleanup(rc);
-        }
+        sched[(size_t)(3 * k)] = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(M_PI * frac)));
+        sched[(size_t)(3 * k + 1)] = (float)(1.0 - std::pow((double)s->cfg.beta1, (double)(t0 + k + 1)));
+        sched[(size_t)(3 * k + 2)] = (float)(1.0 - std::pow((double)s->cfg.beta2, (double)(t0 + k + 1)));
     }
-    s->partial = keep_partial;
+    int* d_idx = nullptr;
+    float *d_part = nullptr, *d_sched = nullptr;
+    int64_t* d_step = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float* keep_partial = s->partial;
+    auto cleanup = [&](int rc) {
+        (void)hipStreamSynchronize(s->stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        (void)hipFree(d_idx);
+        (void)hipFree(d_part);
+        (void)hipFree(d_sched);
+        (void)hipFree(d_step);
+        s->partial = keep_partial;
+        s->sr = StepRef{};
+        return rc;
+    };
+    if (hipMalloc(&d_idx, idx.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_part, (size_t)steps * s->nblk_loss * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d_sched, sched.size() * sizeof(float)) != hipSuccess || hipMalloc(&d_step, sizeof(int64_t)) != hipSuccess)
+        return cleanup(err(AFX_E_NOMEM, "device allocation for the training schedule failed"));
+    if (hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(d_sched, sched.data(), sched.size() * sizeof(float), hipMemcpyHostToDevice, s->stream) !=
+            hipSuccess ||
+        hipMemsetAsync(d_step, 0, sizeof(int64_t), s->stream) != hipSuccess)
+        return cleanup(err(AFX_E_HIP, "training schedule upload failed"));
+    s->partial = d_part;
+    s->sr = StepRef{d_step, (int64_t)B, (int64_t)s->nblk_loss, d_sched};
+    // One step: forward, loss, backward, AdamW, then the device step counter.
+    auto enqueue_step = [&]() -> int {
+        int rc = step_grad(s, s->xd, s->td, s->wd, d_idx, nullptr);
+        if (!rc) rc = adam(s, 0.0f, true);
+        if (rc) return rc;
+        k_step_inc<<<1, 64, 0, s->stream>>>(d_step);
+        HIP_CHECK(hipGetLastError());
+        return AFX_OK;
+    };
+    // Steps are replayed from a captured graph of kChunk steps; the rest run eagerly.
+    constexpr int64_t kChunk = 32;
+    const int64_t nfull = (s->cfg.flags & AFX_MLP_NO_GRAPH) ? 0 : steps / kChunk;
+    if (nfull > 0) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return cleanup(err(AFX_E_HIP, "graph capture failed to start"));
+        int rc = AFX_OK;
+        for (int64_t k = 0; k < kChunk && !rc; ++k) rc = enqueue_step();
+        const hipError_t ce = hipStreamEndCapture(s->stream, &g);
+        if (rc || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return cleanup(rc ? rc : err(AFX_E_HIP, "graph capture failed"));
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return cleanup(err(AFX_E_HIP, "graph instantiation failed"));
+    }
+    const auto w0 = std::chrono::steady_clock::now();
+    if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
+    for (int64_t c = 0; c < nfull; ++c)
+        if (hipGraphLaunch(exec, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "graph launch failed"));
+    for (int64_t k = nfull * kChunk; k < steps; ++k)
+        if (int rc = enqueue_step()) return cleanup(rc);
     if (hipEventRecord(s->ev1, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
     if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "training failed on the device"));
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    s->t = t0 + steps;
     s->st.last_train_gpu_ms = ms;
-    s->st.last_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->st.last_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     s->st.train_steps = s->t;
     if (loss_out) {
         std::vector<float> part((size_t)steps * s->nblk_loss);

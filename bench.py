@@ -67,7 +67,7 @@ def batched_leg(steps):
     return res
 
 
-def surrogate_leg(train_steps):
+def surrogate_leg(train_steps, eager=False):
     """Learned surrogate in the furnished room (generator seed fixed): 64
     sources x 64 receivers = 4096 FDTD impulse responses for training and
     8 x 64 = 512 held out, positions uniform over free cells, each trace
@@ -81,7 +81,7 @@ def surrogate_leg(train_steps):
     rng = np.random.Generator(np.random.PCG64(5))
     src = surrogate.free_cells(w, rng, 72)
     rcv = [surrogate.free_cells(w, rng, 64) for _ in range(72)]
-    sur = surrogate.Surrogate(w, seed=1)
+    sur = surrogate.Surrogate(w, seed=1, flags=1 if eager else 0)  # 1: AFX_MLP_NO_GRAPH
     try:
         t = time.perf_counter()
         ir = sur.simulate(src, rcv)
@@ -180,6 +180,7 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-steps", type=int, default=20000)
     ap.add_argument("--surrogate-steps", type=int, default=30000)
+    ap.add_argument("--surrogate-eager", action="store_true", help="train without captured graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -255,7 +256,7 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_steps)
     if not args.no_surrogate:
-        out["surrogate"] = surrogate_leg(args.surrogate_steps)
+        out["surrogate"] = surrogate_leg(args.surrogate_steps, args.surrogate_eager)
     if not args.no_large:
         out["large"] = large_leg(args.large_steps)
     if not args.no_cpu_baseline:

@@ -48,7 +48,11 @@ typedef struct afx_mlp_config {
     uint32_t seed;        /* initial weights */
     float freq_min, freq_max;
     float beta1, beta2, eps, weight_decay;  /* Adam(W) */
+    uint32_t flags;       /* AFX_MLP_NO_GRAPH: launch every training step eagerly */
 } afx_mlp_config;
+
+/* Training replays captured hipGraphs of 32 steps unless this flag is set. */
+#define AFX_MLP_NO_GRAPH 1u
 
 typedef struct afx_mlp_stats {
     int64_t params;

@@ -119,3 +119,21 @@ def test_mlp_first_adam_step():
     step = (p0 - p1).astype(np.float64)
     assert np.allclose(step[big], lr * np.sign(g[big]), rtol=1e-3, atol=0)
     assert np.all(np.abs(step[~big]) <= lr * 1.001)
+
+
+def test_mlp_graph_replay_matches_eager_training():
+    print("\n[case] mlp training 100 steps, captured graphs vs eager launches", flush=True)
+    from anc_field_py import _afx
+    rng = np.random.default_rng(6)
+    x, t = _toy(256, rng)
+    w = (1.0 / (64 * np.sum(t.astype(np.float64) ** 2, axis=1))).astype(np.float32)
+    kw = dict(in_dim=2, n_freq=8, width=64, depth=3, out_dim=64, batch=64, seed=8, freq_min=0.5, freq_max=4.0)
+    out = []
+    for flags in (0, _afx.MLP_NO_GRAPH):
+        with _afx.Mlp(flags=flags, **kw) as m:
+            m.set_data(x, t, w)
+            loss = m.train(100, 2e-3, 1e-4, seed=4)  # 3 graph replays of 32 steps + 4 eager steps
+            loss2 = m.train(40, 1e-3, 1e-4, seed=5)  # continues: Adam step count and moments carry over
+            out.append((m.params(), loss, loss2))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
