@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -173,8 +174,9 @@ struct afx_sim {
     int64_t total_cells = 0;
     DBuf P[3];
     int rot[3] = {0, 1, 2};
-    DBuf d_rooms, d_tiles, d_mask, d_e_xyz, d_sp_xyz, d_pml_rec, d_pml_side;
+    DBuf d_rooms, d_tiles, d_colmask, d_e_xyz, d_sp_xyz, d_pml_rec, d_pml_side;
     DBuf d_work, d_e_pack, d_pml_nbd;
+    std::vector<std::pair<int64_t, int32_t>> by_cell;  // (cell, entry) sorted by cell
     int32_t nwork = 0;
     int32_t npml = 0;
     int32_t ntiles = 0;
@@ -339,6 +341,42 @@ int build_tile_tables(afx_sim* s)
 
 uint32_t pack_xyz(int x, int y, int z) { return (uint32_t)x | ((uint32_t)y << 11) | ((uint32_t)z << 22); }
 
+// Position of a cell's tile in XCD-major order (the XCD of its slot, then the
+// slot's place in that XCD's sequence), for the current tile shape.
+struct TileRanker {
+    TileShape ts{};
+    std::vector<int32_t> slot_of;
+    std::vector<int64_t> roff;
+    int64_t per = 0, nslot = 0;
+    const afx_sim* s = nullptr;
+    explicit TileRanker(afx_sim* sim) : s(sim)
+    {
+        std::vector<int4> table;
+        std::vector<int32_t> los;
+        build_tiles(sim, table, los);
+        ts = kTiles[sim->tile];
+        nslot = (int64_t)table.size();
+        per = nslot / 8;
+        slot_of.assign(std::max<size_t>(table.size(), 1), -1);
+        for (size_t i = 0; i < los.size(); ++i)
+            if (los[i] >= 0) slot_of[(size_t)los[i]] = (int32_t)i;
+        roff.assign(sim->rooms.size() + 1, 0);
+        for (size_t ri = 0; ri < sim->rooms.size(); ++ri) {
+            const Room& r = sim->rooms[ri];
+            roff[ri + 1] = roff[ri] + (int64_t)((r.nx + ts.tx - 1) / ts.tx) * ((r.ny + ts.ty - 1) / ts.ty) *
+                                          ((r.nz + ts.tz - 1) / ts.tz);
+        }
+    }
+    int64_t rank(int room, int x, int y, int z) const
+    {
+        const Room& r = s->rooms[(size_t)room];
+        const int64_t ntx = (r.nx + ts.tx - 1) / ts.tx, nty = (r.ny + ts.ty - 1) / ts.ty;
+        const int64_t L = roff[(size_t)room] + ((int64_t)(z / ts.tz) * nty + y / ts.ty) * ntx + x / ts.tx;
+        const int64_t sl = slot_of[(size_t)L];
+        return (sl % 8) * per + sl / 8;
+    }
+};
+
 // Disjoint boxes covering the union of the PML slabs: each slab minus the
 // boxes already placed.
 void subtract_box(const std::array<int, 6>& a, const std::array<int, 6>& b, std::vector<std::array<int, 6>>& out)
@@ -453,10 +491,9 @@ int build_special(afx_sim* s)
         if (edge) {
             b = -2;
         } else {
-            auto it = std::lower_bound(s->entries.begin(), s->entries.end(), q.cell,
-                                       [](const Entry& e, int64_t c) { return e.cell < c; });
-            if (it != s->entries.end() && it->cell == q.cell) {
-                b = (int32_t)(it - s->entries.begin());
+            auto it = std::lower_bound(s->by_cell.begin(), s->by_cell.end(), std::make_pair(q.cell, (int32_t)-1));
+            if (it != s->by_cell.end() && it->first == q.cell) {
+                b = it->second;
                 info[(size_t)b] |= kInfoSpecial;
             } else if (s->pml_mode == 2) {
                 // a PML cell: its value (and psi) come from the PML update; the
@@ -511,9 +548,8 @@ int build_special(afx_sim* s)
                     if (!(x < lo[0] || x >= hi[0] || y < lo[1] || y >= hi[1] || z < lo[2] || z >= hi[2])) continue;
                     const int64_t c = r.base + ((int64_t)z * r.ny + y) * r.nx + x;
                     int32_t flag = -1;
-                    auto it = std::lower_bound(s->entries.begin(), s->entries.end(), c,
-                                               [](const Entry& e, int64_t v) { return e.cell < v; });
-                    if (it != s->entries.end() && it->cell == c) {
+                    auto it = std::lower_bound(s->by_cell.begin(), s->by_cell.end(), std::make_pair(c, (int32_t)-1));
+                    if (it != s->by_cell.end() && it->first == c) {
                         flag = -2;
                     } else {
                         auto q = std::lower_bound(pml_sp.begin(), pml_sp.end(), std::make_pair(c, (int32_t)-1));
@@ -539,24 +575,9 @@ int build_special(afx_sim* s)
         const TileShape ts = kTiles[s->tile];
         const int NT = ts.tx * ts.ty;
         const size_t nslot = table.size();
-        std::vector<int32_t> slot_of(std::max<size_t>(nslot, 1), -1);
-        for (size_t i = 0; i < los.size(); ++i)
-            if (los[i] >= 0) slot_of[(size_t)los[i]] = (int32_t)i;
-        std::vector<int64_t> roff(s->rooms.size() + 1, 0);
-        for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
-            const Room& r = s->rooms[ri];
-            roff[ri + 1] = roff[ri] + (int64_t)((r.nx + ts.tx - 1) / ts.tx) * ((r.ny + ts.ty - 1) / ts.ty) *
-                                          ((r.nz + ts.tz - 1) / ts.tz);
-        }
-        // rank of a slot in XCD-major order: XCD slot % 8, then slot / 8
-        const int64_t per = (int64_t)nslot / 8;
-        auto rank_of_cell = [&](int room, int x, int y, int z) -> int64_t {
-            const Room& r = s->rooms[(size_t)room];
-            const int64_t ntx = (r.nx + ts.tx - 1) / ts.tx, nty = (r.ny + ts.ty - 1) / ts.ty;
-            const int64_t L = roff[(size_t)room] + ((int64_t)(z / ts.tz) * nty + y / ts.ty) * ntx + x / ts.tx;
-            const int64_t sl = slot_of[(size_t)L];
-            return (sl % 8) * per + sl / 8;
-        };
+        const TileRanker ranker(s);
+        const int64_t per = ranker.per;
+        auto rank_of_cell = [&](int room, int x, int y, int z) { return ranker.rank(room, x, y, z); };
         std::vector<int32_t> bcount(nslot + 1, 0), pcount(nslot + 1, 0);
         std::vector<int32_t> brank(s->entries.size()), prank(pcell.size());
         for (size_t e = 0; e < s->entries.size(); ++e) {
@@ -691,7 +712,33 @@ int build_special(afx_sim* s)
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
     s->npml = (int32_t)pcell.size();
-    HIP_TRY(upload(s->d_mask, mask, s->stream));
+    // Column masks of the interior tiles: for tile slot t and tile thread
+    // (x, y), bit z - z0 is set when cell (x, y, z) is left to a list.
+    {
+        std::vector<int4> table;
+        std::vector<int32_t> los;
+        build_tiles(s, table, los);
+        const TileShape ts = kTiles[s->tile];
+        const int NT = ts.tx * ts.ty;
+        std::vector<uint32_t> col(std::max<size_t>(table.size() * (size_t)NT, 1), 0u);
+        for (size_t sl = 0; sl < table.size(); ++sl) {
+            const int4 t4 = table[sl];
+            if (t4.x < 0) continue;
+            const Room& r = s->rooms[(size_t)t4.x];
+            for (int th = 0; th < NT; ++th) {
+                const int x = t4.y + th % ts.tx, y = t4.z + th / ts.tx;
+                if (x >= r.nx || y >= r.ny) continue;
+                uint32_t bits = 0;
+                for (int dz = 0; dz < ts.tz && t4.w + dz < r.nz; ++dz) {
+                    const int64_t c = r.base + ((int64_t)(t4.w + dz) * r.ny + y) * r.nx + x;
+                    if ((mask[(size_t)(c >> 5)] >> (c & 31)) & 1u) bits |= 1u << dz;
+                }
+                col[sl * (size_t)NT + (size_t)th] = bits;
+            }
+        }
+        HIP_TRY(upload(s->d_colmask, col, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
     // sources
     int64_t cap = 1;
     for (auto& q : s->srcs) cap = std::max<int64_t>(cap, (int64_t)q.samples.size());
@@ -877,6 +924,27 @@ int commit(afx_sim* s)
             if (rc) return rc;
         }
     std::sort(s->entries.begin(), s->entries.end(), [](const Entry& a, const Entry& b) { return a.cell < b.cell; });
+    // Entries (and so their IIR rings) are kept in the order the fused kernel
+    // walks them for the tile shape in force at commit: by tile, then by cell.
+    // Chunks then touch runs of consecutive ring slots.  by_cell finds an
+    // entry from its cell.
+    {
+        const TileRanker ranker(s);
+        std::vector<int64_t> rk(s->entries.size());
+        for (size_t e = 0; e < s->entries.size(); ++e) {
+            const Entry& en = s->entries[e];
+            rk[e] = ranker.rank(en.room, en.x, en.y, en.z);
+        }
+        std::vector<size_t> order(s->entries.size());
+        std::iota(order.begin(), order.end(), (size_t)0);
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return rk[a] < rk[b]; });
+        std::vector<Entry> sorted(s->entries.size());
+        for (size_t k = 0; k < order.size(); ++k) sorted[k] = s->entries[order[k]];
+        s->entries.swap(sorted);
+        s->by_cell.resize(s->entries.size());
+        for (size_t e = 0; e < s->entries.size(); ++e) s->by_cell[e] = {s->entries[e].cell, (int32_t)e};
+        std::sort(s->by_cell.begin(), s->by_cell.end());
+    }
     s->nb = (int32_t)s->entries.size();
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<RoomDev> rd;
@@ -1060,7 +1128,7 @@ StepArgs make_args(afx_sim* s)
     a.nwork = cosched ? s->nwork : 0;
     a.e_pack = s->d_e_pack.as<int4>();
     a.pml_nbd = s->d_pml_nbd.as<uint32_t>();
-    a.mask = s->d_mask.as<uint32_t>();
+    a.colmask = s->d_colmask.as<uint32_t>();
     a.e_xyz = s->d_e_xyz.as<uint32_t>();
     a.sp_xyz = s->d_sp_xyz.as<uint32_t>();
     a.dt = s->cfg.dt;

@@ -89,7 +89,7 @@ struct StepArgs {
     int32_t t_rel;              // step offset inside the chunk
     // fused step kernel: cells finished outside the interior tiles, and the
     // packed coordinates (x | y << 11 | z << 22) of entries and special cells
-    const uint32_t* mask;
+    const uint32_t* colmask;    // per tile slot and tile thread: bit z - z0 marks a cell finished by a list
     const uint32_t* e_xyz;
     const uint32_t* sp_xyz;
     // physical PML state (compute type): phi and psi double buffers (the

@@ -116,13 +116,13 @@ __device__ __forceinline__ bool in_pml(const RoomDev& r, int x, int y, int z)
 // One thread owns an (x, y) column of a TX x TY x TZ tile and marches z,
 // carrying p^n at z-1, z, z+1 in registers; x and y neighbours come from the
 // same and adjacent 64-lane rows (L1 / L2 hits).  The sum order is the
-// reference's: x+1, x-1, y+1, y-1, z+1, z-1.  With MASK, cells whose bit is
-// set in `mask` (boundary, source/receiver and PML cells, finished by other
+// reference's: x+1, x-1, y+1, y-1, z+1, z-1.  Cells whose bit z - z0 is set
+// in `skipbits` (boundary, source/receiver and PML cells, finished by other
 // threads of the same launch) are left alone.
-template <typename S, typename C, int TZ, bool MASK>
+template <typename S, typename C, int TZ>
 __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __restrict__ cur,
                                                 const S* __restrict__ prev, const RoomDev& r, int x, int y,
-                                                int z0, C l2, C c2, const uint32_t* __restrict__ mask)
+                                                int z0, C l2, C c2, uint32_t skipbits)
 {
     if (x < 1 || x >= r.nx - 1 || y < 1 || y >= r.ny - 1) return;
     const int zs = z0 < 1 ? 1 : z0;
@@ -130,12 +130,14 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
     if (zs >= ze) return;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + x;
-    // Software pipeline with two register sets A and B used alternately: the
-    // loads of plane z+1 are issued before plane z is computed and stored,
-    // and no loaded register is copied while its load may be in flight.
+    // Software pipeline over a ring of kDepth register sets: the loads of
+    // plane z+kDepth are issued right after plane z is computed, so kDepth
+    // planes are in flight per thread.  Prefetch addresses clamp to the last
+    // plane, so no load sits behind a branch; a plane past the end is not
+    // computed (a uniform branch around arithmetic and the store only).
+    constexpr int kDepth = 4;
     struct Plane {
         C u, xp, xm, yp, ym, pm;  // p^n at z+1, x+-1, y+-1; p^{n-1}
-        uint32_t mw;
     };
     auto load_plane = [&](Plane& P, int64_t j) {
         P.u = ld<S, C>(cur, j + sz);
@@ -144,42 +146,35 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
         P.yp = ld<S, C>(cur, j + sy);
         P.ym = ld<S, C>(cur, j - sy);
         P.pm = ld<S, C>(prev, j);
-        P.mw = MASK ? mask[j >> 5] : 0u;
     };
-    // (pd, pc) = p^n at z-1 and z of the plane being computed
-    auto do_plane = [&](const Plane& P, int64_t j, C pd, C pc) {
-        C s = P.xp + P.xm;
-        s = s + P.yp;
-        s = s + P.ym;
-        s = s + P.u;
-        s = s + pd;
-        const bool skip = MASK && ((P.mw >> (j & 31)) & 1u);
-        if (!skip) {
-            const C v = l2 * s + c2 * pc;
-            st<S, C>(out, j, v - P.pm);
-        }
-    };
-    // Planes go in pairs (z, z1 = min(z+1, ze-1)); with an odd count the last
-    // plane is computed twice, storing the same value, and prefetches clamp
-    // to the last plane, so the loop body has no branch around a load.
-    C pd = ld<S, C>(cur, i - sz);
-    C pc = ld<S, C>(cur, i);
     const int64_t i_last = i + (int64_t)(ze - 1 - zs) * sz;
-    Plane A, B;
-    load_plane(A, i);
-    for (int z = zs; z < ze; z += 2, i += 2 * sz) {
-        const bool odd_tail = z + 1 >= ze;  // uniform across the wave
-        const int64_t i1 = odd_tail ? i : i + sz;
-        const int64_t i2 = (z + 2 < ze) ? i + 2 * sz : i_last;
-        load_plane(B, i1);
-        do_plane(A, i, pd, pc);
-        const C au = A.u;  // p^n at z+1, already consumed above
-        load_plane(A, i2);
-        const C pd1 = odd_tail ? pd : pc;
-        const C pc1 = odd_tail ? pc : au;
-        do_plane(B, i1, pd1, pc1);
-        pd = pc1;
-        pc = B.u;
+    C pd = ld<S, C>(cur, i - sz);  // p^n at z-1 and z of the next plane to compute
+    C pc = ld<S, C>(cur, i);
+    Plane P[kDepth];
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) load_plane(P[d], zs + d < ze ? i + (int64_t)d * sz : i_last);
+    for (int z = zs; z < ze; z += kDepth) {
+#pragma unroll
+        for (int d = 0; d < kDepth; ++d) {
+            const int64_t j = i + (int64_t)d * sz;
+            if (z + d < ze) {  // uniform across the wave
+                C s = P[d].xp + P[d].xm;
+                s = s + P[d].yp;
+                s = s + P[d].ym;
+                s = s + P[d].u;
+                s = s + pd;
+                const bool skip = (skipbits >> (z + d - z0)) & 1u;
+                if (!skip) {
+                    const C v = l2 * s + c2 * pc;
+                    st<S, C>(out, j, v - P[d].pm);
+                }
+                pd = pc;
+                pc = P[d].u;
+            }
+            const int64_t jn = z + d + kDepth < ze ? j + (int64_t)kDepth * sz : i_last;
+            load_plane(P[d], jn);
+        }
+        i += (int64_t)kDepth * sz;
     }
 }
 
@@ -195,7 +190,7 @@ __global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const
     const RoomDev r = rooms[t.x];
     const int x = t.y + (int)(threadIdx.x % TX);
     const int y = t.z + (int)(threadIdx.x / TX);
-    interior_column<S, C, TZ, false>(out, cur, prev, r, x, y, t.w, l2, c2, nullptr);
+    interior_column<S, C, TZ>(out, cur, prev, r, x, y, t.w, l2, c2, 0u);
 }
 
 // ---------------------------------------------------------------- boundary
@@ -696,9 +691,8 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary(BndCtx<S, C> c, int nb
 // The latency-bound lists come first so they overlap the interior stream.
 template <typename S, int TX, int TY, int TZ>
 __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>::T> c,
-                                                   const int4* __restrict__ tiles, int ntiles, int nb, int nblk_b,
-                                                   int nsp, int nblk_s, int npml, int nblk_p, int tile0,
-                                                   const uint32_t* __restrict__ mask,
+                                                   const int4* __restrict__ tiles,
+                                                   const uint32_t* __restrict__ colmask,
                                                    const int64_t* __restrict__ d_t, int t_rel,
                                                    const int4* __restrict__ work,
                                                    const int4* __restrict__ e_pack,
@@ -707,44 +701,62 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
     using C = typename Compute<S>::T;
     constexpr int NT = TX * TY;
     __shared__ C pml_lds[3][NT];
-    int bid = (int)blockIdx.x;
-    if (work) {
-        // Work table: item (kind, a, b).  kind 0: interior tile slot a;
-        // 1: packed boundary entries e_pack[a..b); 2: PML list [a..b);
-        // 3: special cells [a..b); -1: idle.  Items of one XCD (blockIdx % 8) follow
-        // its tiles in order, each list chunk right after the tile holding
-        // its last cell, so both read the same planes of p^n through L2.
-        const int4 wk = work[bid];
-        if (wk.x == 0) {
-            const int4 t4 = tiles[wk.y];
-            const RoomDev r = c.rooms[t4.x];
-            const int x = t4.y + (int)(threadIdx.x % TX);
-            const int y = t4.z + (int)(threadIdx.x / TX);
-            interior_column<S, C, TZ, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, mask);
-            return;
-        }
-        if (wk.x < 0) return;
-        const int64_t t = *d_t + t_rel;
-        if (wk.x == 2) {  // whole workgroup: it meets a barrier
-            pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, pml_lds);
-            return;
-        }
-        const int q = wk.y + (int)threadIdx.x;
-        if (q >= wk.z) return;
-        const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
-        if (wk.x == 1)
-            boundary_packed<S, C>(c, e_pack[q], ixb);
-        else
-            special_entry<S, C>(c, q, ixb, t, 3);
+    // Work table: item (kind, a, b).  kind 0: interior tile slot a;
+    // 1: packed boundary entries e_pack[a..b); 2: PML list [a..b);
+    // 3: special cells [a..b); -1: idle.  Items of one XCD (blockIdx % 8)
+    // follow its tiles in order, each list chunk right after the tile holding
+    // its last cell, so both read the same planes of p^n through L2.
+    const int4 wk = work[blockIdx.x];
+    if (wk.x == 0) {
+        const int4 t4 = tiles[wk.y];
+        const RoomDev r = c.rooms[t4.x];
+        const int x = t4.y + (int)(threadIdx.x % TX);
+        const int y = t4.z + (int)(threadIdx.x / TX);
+        const uint32_t bits = colmask[(int64_t)wk.y * NT + threadIdx.x];
+        interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, bits);
         return;
     }
+    if (wk.x < 0) return;
+    const int64_t t = *d_t + t_rel;
+    if (wk.x == 2) {  // whole workgroup: it meets a barrier
+        pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, pml_lds);
+        return;
+    }
+    const int q = wk.y + (int)threadIdx.x;
+    if (q >= wk.z) return;
+    const unsigned ixb = c.N > 0 ? ring_index(t, c.N) : 0u;
+    if (wk.x == 1)
+        boundary_packed<S, C>(c, e_pack[q], ixb);
+    else
+        special_entry<S, C>(c, q, ixb, t, 3);
+}
+
+// The same step with the lists in leading workgroups of their own
+// (AFX_LIST_BLOCKS):
+//   [0, nblk_b)            boundary entries, one per thread
+//   [.., +nblk_s)          source/receiver cells
+//   [.., +nblk_p)          PML cells (phi, psi and pressure)
+//   [tile0, tile0+ntiles)  interior tiles; tile0 is a multiple of 8 so the
+//                          XCD-swizzled tile order keeps its XCD mapping
+template <typename S, int TX, int TY, int TZ>
+__global__ __launch_bounds__(TX * TY) void k_fused_blocks(BndCtx<S, typename Compute<S>::T> c,
+                                                          const int4* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ colmask, int nb, int nblk_b,
+                                                          int nsp, int nblk_s, int npml, int nblk_p, int tile0,
+                                                          const int64_t* __restrict__ d_t, int t_rel)
+{
+    using C = typename Compute<S>::T;
+    constexpr int NT = TX * TY;
+    const int bid = (int)blockIdx.x;
     if (bid >= tile0) {
-        const int4 t4 = tiles[bid - tile0];
+        const int slot = bid - tile0;
+        const int4 t4 = tiles[slot];
         if (t4.x < 0) return;
         const RoomDev r = c.rooms[t4.x];
         const int x = t4.y + (int)(threadIdx.x % TX);
         const int y = t4.z + (int)(threadIdx.x / TX);
-        interior_column<S, C, TZ, true>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, mask);
+        const uint32_t bits = colmask[(int64_t)slot * NT + threadIdx.x];
+        interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, bits);
         return;
     }
     const int64_t t = *d_t + t_rel;
@@ -1006,17 +1018,16 @@ static hipError_t fused_t(const StepArgs& a, hipStream_t s)
     constexpr int NT = TX * TY;
     BndCtx<S, C> c = make_ctx<S, C>(a);
     if (a.work) {
-        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.ntiles, a.nb, 0,
-                           a.nsp, 0, a.npml, 0, 0, a.mask, a.d_t, a.t_rel, a.work, a.e_pack, a.pml_nbd);
+        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.colmask, a.d_t,
+                           a.t_rel, a.work, a.e_pack, a.pml_nbd);
         return hipGetLastError();
     }
     const int nblk_b = (a.nb + NT - 1) / NT;
     const int nblk_s = (a.nsp + NT - 1) / NT;
     const int nblk_p = (a.npml + NT - 1) / NT;
     const int tile0 = (nblk_b + nblk_s + nblk_p + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles, a.ntiles, a.nb,
-                       nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.mask, a.d_t, a.t_rel, nullptr, nullptr,
-                       nullptr);
+    hipLaunchKernelGGL((k_fused_blocks<S, TX, TY, TZ>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles,
+                       a.colmask, a.nb, nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.d_t, a.t_rel);
     return hipGetLastError();
 }
 
