@@ -90,6 +90,7 @@ _SIGS = {
     "afx_classify": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32, C.POINTER(C.c_int64), _i32p]),
     "afx_get_receivers": (C.c_int, [_P, _f32p, C.c_int64, C.POINTER(C.c_int64)]),
     "afx_get_pressure": (C.c_int, [_P, C.c_int32, C.c_int32, _f32p, C.c_int64]),
+    "afx_get_plane": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f32p, C.c_int64]),
     "afx_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "afx_autotune": (C.c_int, [_P, C.c_int32, _i32p]),
     "afx_profile_run": (C.c_int, [_P, C.c_int64, C.POINTER(Profile)]),
@@ -298,6 +299,15 @@ class Sim:
         out = np.empty(int(np.prod(dims)), np.float32)
         _check(lib.afx_get_pressure(self._h, room, which, out.ctypes.data_as(_f32p), out.size))
         return out.reshape(dims)
+
+    def plane(self, axis, index, dims, which=0, room=0):
+        """FP32 plane of room `room` (dims = (nz, ny, nx)): axis 0 -> [nz][ny],
+        1 -> [nz][nx] (the reference's cross section), 2 -> [ny][nx]."""
+        nz, ny, nx = dims
+        shape = {0: (nz, ny), 1: (nz, nx), 2: (ny, nx)}[axis]
+        out = np.empty(shape, np.float32)
+        _check(lib.afx_get_plane(self._h, room, axis, index, which, out.ctypes.data_as(_f32p), out.size))
+        return out
 
     def room_dims(self, room):
         raise NotImplementedError("pass dims for rooms other than 0")

@@ -1717,6 +1717,39 @@ int afx_get_pressure(afx_sim* s, int32_t room, int32_t which, float* out, int64_
     return AFX_OK;
 }
 
+int afx_get_plane(afx_sim* s, int32_t room, int32_t axis, int32_t index, int32_t which, float* out, int64_t cap)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!s->committed) return fail(AFX_E_STATE, "no field before the first run");
+    if (room < 0 || room >= (int32_t)s->rooms.size()) return fail(AFX_E_ARG, "room %d does not exist", room);
+    if (which < 0 || which > 2) return fail(AFX_E_ARG, "which must be 0, 1 or 2");
+    if (axis < 0 || axis > 2) return fail(AFX_E_ARG, "axis must be 0 (x), 1 (y) or 2 (z)");
+    const Room& r = s->rooms[(size_t)room];
+    const int dims[3] = {r.nx, r.ny, r.nz};
+    if (index < 0 || index >= dims[axis])
+        return fail(AFX_E_RANGE, "plane %d outside 0..%d on axis %d", index, dims[axis] - 1, axis);
+    const int64_t sx = 1, sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    int nu, nv;
+    int64_t su, sv, base;
+    if (axis == 0) {  // [z][y]
+        nu = r.ny, nv = r.nz, su = sy, sv = sz, base = index * sx;
+    } else if (axis == 1) {  // [z][x], the reference's cross section
+        nu = r.nx, nv = r.nz, su = sx, sv = sz, base = index * sy;
+    } else {  // [y][x]
+        nu = r.nx, nv = r.ny, su = sx, sv = sy, base = index * sz;
+    }
+    const int64_t n = (int64_t)nu * nv;
+    if (!out || cap < n) return fail(AFX_E_ARG, "buffer holds %lld values, %lld needed", (long long)cap, (long long)n);
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    DBuf tmp;
+    HIP_TRY(tmp.alloc(sizeof(float) * (size_t)n));
+    const char* src = static_cast<const char*>(s->P[s->rot[which]].p) + prec_bytes(s->prec) * (size_t)r.base;
+    HIP_TRY(launch_plane_f32(src, s->prec, tmp.as<float>(), nu, nv, su, sv, base, s->scale, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, tmp.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return AFX_OK;
+}
+
 int afx_voxelize_kerkythea(const char* path, float res, int32_t dims[3], int32_t* vox, int64_t vox_cap, float* a,
                            float* b, int32_t coef_cap, int32_t* ncoef, int32_t* nmat)
 {

@@ -147,5 +147,7 @@ hipError_t launch_pml_compat(const PmlArgs& a, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t s);
 hipError_t launch_to_f32(const void* src, Prec prec, float* dst, int64_t n, double scale, hipStream_t s);
+hipError_t launch_plane_f32(const void* src, Prec prec, float* dst, int nu, int nv, int64_t su, int64_t sv,
+                            int64_t base, double scale, hipStream_t s);
 
 }  // namespace afx

@@ -1001,6 +1001,36 @@ __global__ void k_to_f32(const S* __restrict__ src, float* __restrict__ dst, int
         dst[i] = (float)((C)src[i] * (C)scale);
 }
 
+// One axis-aligned plane of a field, as FP32: dst[v][u] = src[base + u*su + v*sv] * scale.
+// copy_plane.cl:29-34 does the y-plane (u = x, v = z) for the live view.
+template <typename S>
+__global__ void k_plane_f32(const S* __restrict__ src, float* __restrict__ dst, int nu, int nv, int64_t su,
+                            int64_t sv, int64_t base, double scale)
+{
+    using C = typename Compute<S>::T;
+    const int64_t n = (int64_t)nu * nv;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = k % nu, v = k / nu;
+        dst[k] = (float)((C)src[base + u * su + v * sv] * (C)scale);
+    }
+}
+
+hipError_t launch_plane_f32(const void* src, Prec prec, float* dst, int nu, int nv, int64_t su, int64_t sv,
+                            int64_t base, double scale, hipStream_t s)
+{
+    int64_t blocks = ((int64_t)nu * nv + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) return hipSuccess;
+    const dim3 g((unsigned)blocks), b(256);
+    switch (prec) {
+    case Prec::F32: hipLaunchKernelGGL(k_plane_f32<float>, g, b, 0, s, (const float*)src, dst, nu, nv, su, sv, base, scale); break;
+    case Prec::F64: hipLaunchKernelGGL(k_plane_f32<double>, g, b, 0, s, (const double*)src, dst, nu, nv, su, sv, base, scale); break;
+    case Prec::F16: hipLaunchKernelGGL(k_plane_f32<_Float16>, g, b, 0, s, (const _Float16*)src, dst, nu, nv, su, sv, base, scale); break;
+    case Prec::BF16: hipLaunchKernelGGL(k_plane_f32<__bf16>, g, b, 0, s, (const __bf16*)src, dst, nu, nv, su, sv, base, scale); break;
+    }
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- launchers
 template <typename S, int TX, int TY, int TZ>
 static hipError_t interior_t(const StepArgs& a, hipStream_t s)

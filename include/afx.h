@@ -16,6 +16,7 @@
  *   afx_run                              run               main.cpp:117-122, simulation.hpp:342-434
  *   afx_get_receivers                    save / mics()     main.cpp:124-127, simulation.hpp:95
  *   afx_get_pressure                     p()               simulation.hpp:94
+ *   afx_get_plane                        vis cross section copy_plane.cl:29-34, simulation.hpp:286-289,394-401
  *   afx_add_room, afx_*_cell             (no counterpart: batched rooms and cell-addressed setup)
  *   afx_get_stats                        (boundary counts logged at lrs-data.hpp:212-214)
  *   afx_autotune                         (no counterpart: tile-shape search)
@@ -157,6 +158,12 @@ int afx_synchronize(afx_sim* sim);
 int afx_get_receivers(afx_sim* sim, float* out, int64_t cap, int64_t* n_written);
 /* which: 0 = newest pressure p^{n+1}, 1 = p^n, 2 = p^{n-1}; FP32 copy of room `room` */
 int afx_get_pressure(afx_sim* sim, int32_t room, int32_t which, float* out, int64_t cap);
+/* One axis-aligned plane of a pressure field (which as in afx_get_pressure),
+ * FP32: axis 0 gives [nz][ny] at x = index, axis 1 [nz][nx] at y = index
+ * (the cross section copy_plane.cl:29-34 takes for the live view,
+ * simulation.hpp:394-401), axis 2 [ny][nx] at z = index.  Only the plane
+ * crosses PCIe. */
+int afx_get_plane(afx_sim* sim, int32_t room, int32_t axis, int32_t index, int32_t which, float* out, int64_t cap);
 int afx_get_stats(afx_sim* sim, afx_stats* st);
 
 /* Per-kernel timing on the handle's stream: runs `steps` steps eagerly with a

@@ -289,3 +289,27 @@ def test_reduced_precision_error(precision):
     bound = 3e-2 if precision == "fp16" else 2.5e-1
     print(f"{precision} rel L2 vs fp64: {err:.3e}")
     assert err < bound
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_plane_snapshot_matches_field(precision):
+    _case(f"plane snapshots, {precision}")
+    v, a, b, rng = _closed_room(27, dims=(36, 30, 26))
+    sc = rooms.interior_cells(v, rng, 1)
+    steps = 60
+    dt = rooms.courant_dt(0.02)
+    s = afx.Sim(c=rooms.C_AIR, dx=0.02, dt=dt, res=0.02, compat=False, precision=precision)
+    s.set_voxels(v)
+    s.set_materials(a, b)
+    s.add_source_cell(sc[0], rooms.ricker(steps, dt))
+    s.run(steps)
+    dims = v.shape
+    for which in (0, 1):
+        f = s.pressure(which)
+        assert np.array_equal(s.plane(1, 13, dims, which), f[:, 13, :])   # cross section, [z][x]
+        assert np.array_equal(s.plane(0, 7, dims, which), f[:, :, 7])
+        assert np.array_equal(s.plane(2, 11, dims, which), f[11])
+    assert np.abs(s.plane(1, int(sc[0][1]), dims)).max() > 0
+    with pytest.raises(afx.AfxError):
+        s.plane(1, 30, dims)
+    s.close()
