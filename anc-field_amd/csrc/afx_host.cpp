@@ -561,13 +561,12 @@ int build_special(afx_sim* s)
                     mask[(size_t)(c >> 5)] |= 1u << (c & 31);
                 }
     }
-    // Work table of the fused kernel.  Boundary entries (through an index
-    // list; the IIR rings stay in entry order) and PML cells (the list itself;
-    // its state lives in full-grid arrays) are grouped by the interior tile
-    // that holds them, in XCD-major tile order.  Workgroup b runs on XCD
-    // b % 8, so each XCD's sequence is its tiles in order, each followed by
-    // the list chunks whose last cell lies in it; sequences are interleaved
-    // and padded with idle items.
+    // Work table of the fused kernel.  Boundary entries (packed records; the
+    // IIR rings stay in entry order) and PML cells (the list itself) are
+    // grouped by the interior tile that holds them, in XCD-major tile order.
+    // Workgroup b runs on XCD b % 8, so each XCD's sequence is its tiles in
+    // order, each preceded by the list chunks whose first cell lies in it;
+    // sequences are interleaved and padded with idle items.
     {
         std::vector<int4> table;
         std::vector<int32_t> los;
@@ -616,15 +615,19 @@ int build_special(afx_sim* s)
             auto& v = seq[(size_t)x];
             int32_t bcur = bcount[(size_t)(x * per)], pcur = pcount[(size_t)(x * per)];
             const int32_t bend = bcount[(size_t)((x + 1) * per)], pend = pcount[(size_t)((x + 1) * per)];
+            // a chunk goes just before the tile holding its first cell: the
+            // latency-bound chunks then overlap the streaming tiles instead
+            // of trailing them, and whichever of the two comes first brings
+            // the shared planes into L2
             for (int64_t k = 0; k < per; ++k) {
                 const int64_t sl = k * 8 + x;
-                if (table[(size_t)sl].x >= 0) v.push_back(make_int4(0, (int)sl, 0, 0));
                 const int64_t rk = x * per + k;
-                for (; bcount[(size_t)rk + 1] - bcur >= NT; bcur += NT) v.push_back(make_int4(1, bcur, bcur + NT, 0));
-                for (; pcount[(size_t)rk + 1] - pcur >= NT; pcur += NT) v.push_back(make_int4(2, pcur, pcur + NT, 0));
+                for (; bcur < bcount[(size_t)rk + 1]; bcur += NT)
+                    v.push_back(make_int4(1, bcur, std::min<int32_t>(bcur + NT, bend), 0));
+                for (; pcur < pcount[(size_t)rk + 1]; pcur += NT)
+                    v.push_back(make_int4(2, pcur, std::min<int32_t>(pcur + NT, pend), 0));
+                if (table[(size_t)sl].x >= 0) v.push_back(make_int4(0, (int)sl, 0, 0));
             }
-            if (bcur < bend) v.push_back(make_int4(1, bcur, bend, 0));
-            if (pcur < pend) v.push_back(make_int4(2, pcur, pend, 0));
         }
         // packed boundary records in work order
         std::vector<int4> epack(tb.size());
