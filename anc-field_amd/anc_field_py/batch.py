@@ -29,14 +29,21 @@ def make_batch(workloads, precision="fp32", compat=False, **kw):
     return s
 
 
-def run_batched(workloads, steps, precision="fp32", **kw):
-    """Returns (traces [sum of receivers, steps], wall seconds incl. setup)."""
+def run_batched(workloads, steps, precision="fp32", tune=False, **kw):
+    """Returns (traces [sum of receivers, steps], wall seconds incl. setup,
+    tile shape).  With tune, afx_autotune picks the tile shape first (every
+    candidate checked bit-exact); tuning time is not included."""
     t0 = time.perf_counter()
     s = make_batch(workloads, precision, **kw)
+    tile = None
+    if tune:
+        t1 = time.perf_counter()
+        tile = s.autotune(30)
+        t0 += time.perf_counter() - t1
     s.run(steps)
     tr = s.receivers()
     s.close()
-    return tr, time.perf_counter() - t0
+    return tr, time.perf_counter() - t0, tile
 
 
 def run_sequential(workloads, steps, precision="fp32", **kw):

@@ -174,7 +174,7 @@ struct afx_sim {
     int64_t total_cells = 0;
     DBuf P[3];
     int rot[3] = {0, 1, 2};
-    DBuf d_rooms, d_tiles, d_colmask, d_e_xyz, d_sp_xyz, d_pml_rec, d_pml_side;
+    DBuf d_rooms, d_tiles, d_colmask, d_pml_rec, d_pml_side;
     DBuf d_work, d_e_pack, d_pml_nbd;
     std::vector<std::pair<int64_t, int32_t>> by_cell;  // (cell, entry) sorted by cell
     int32_t nwork = 0;
@@ -475,7 +475,6 @@ int build_special(afx_sim* s)
     std::vector<uint32_t> info((size_t)s->nb);
     for (int32_t e = 0; e < s->nb; ++e) info[(size_t)e] = s->entries[(size_t)e].info & ~kInfoSpecial;
     std::vector<int32_t> cell, room, bidx, soff(1, 0), sids, moff(1, 0), mids;
-    std::vector<uint32_t> xyz;
     // cells the fused kernel's interior tiles leave to other threads: every
     // boundary cell and every source/receiver cell
     std::vector<uint32_t> mask((size_t)((s->total_cells + 31) / 32), 0u);
@@ -485,7 +484,6 @@ int build_special(afx_sim* s)
         mask[(size_t)(q.cell >> 5)] |= 1u << (q.cell & 31);
         cell.push_back((int32_t)q.cell);
         room.push_back(q.room);
-        xyz.push_back(pack_xyz(q.x, q.y, q.z));
         int32_t b = -1;
         const bool edge = q.x == 0 || q.y == 0 || q.z == 0 || q.x == r.nx - 1 || q.y == r.ny - 1 || q.z == r.nz - 1;
         if (edge) {
@@ -522,7 +520,6 @@ int build_special(afx_sim* s)
     HIP_TRY(upload(s->d_sp_mic_off, moff, s->stream));
     HIP_TRY(upload(s->d_sp_mic, mids, s->stream));
     HIP_TRY(upload(s->d_e_info, info, s->stream));
-    HIP_TRY(upload(s->d_sp_xyz, xyz, s->stream));
     // fused path: every cell of each room's PML slabs (solid ones included)
     // is one PML thread; boundary cells there only advance phi and psi, PML
     // source/receiver cells are finished by their PML thread
@@ -982,16 +979,14 @@ int commit(afx_sim* s)
     }
     HIP_TRY(upload(s->d_rooms, rd, s->stream));
     std::vector<int32_t> ecell((size_t)s->nb);
-    std::vector<uint32_t> emat((size_t)s->nb), exyz((size_t)s->nb);
+    std::vector<uint32_t> emat((size_t)s->nb);
     for (int32_t e = 0; e < s->nb; ++e) {
         const Entry& en = s->entries[(size_t)e];
         ecell[(size_t)e] = (int32_t)en.cell;
         emat[(size_t)e] = en.mat;
-        exyz[(size_t)e] = pack_xyz(en.x, en.y, en.z);
     }
     HIP_TRY(upload(s->d_e_cell, ecell, s->stream));
     HIP_TRY(upload(s->d_e_mat, emat, s->stream));
-    HIP_TRY(upload(s->d_e_xyz, exyz, s->stream));
     const size_t cbytes = s->prec == Prec::F64 ? 8 : 4;
     const int N = std::max(s->ncoef, 1);
     const size_t rbytes = cbytes * (size_t)3 * N * (size_t)std::max(s->nb, 1);
@@ -1132,8 +1127,6 @@ StepArgs make_args(afx_sim* s)
     a.e_pack = s->d_e_pack.as<int4>();
     a.pml_nbd = s->d_pml_nbd.as<uint32_t>();
     a.colmask = s->d_colmask.as<uint32_t>();
-    a.e_xyz = s->d_e_xyz.as<uint32_t>();
-    a.sp_xyz = s->d_sp_xyz.as<uint32_t>();
     a.dt = s->cfg.dt;
     a.dx = s->cfg.dx;
     a.c = s->cfg.c;
@@ -1994,16 +1987,23 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
             if (cand == base_tile) continue;
             if (int rc = snap_restore(s, sn)) return rc;
             if (int rc = set_tile(s, tl)) return rc;
-            // untimed pass (captures graphs, warms caches), then the check pass
+            // untimed pass (captures graphs, warms caches), then three timed
+            // passes from the same state; the first is also the check pass
             if (int rc = run_steps(s, steps_per_trial)) return rc;
-            if (int rc = snap_restore(s, sn)) return rc;
-            HIP_TRY(hipEventRecord(e0, s->stream));
-            if (int rc = run_steps(s, steps_per_trial)) return rc;
-            HIP_TRY(hipEventRecord(e1, s->stream));
-            HIP_TRY(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            HIP_TRY(hipMemcpy(cand < 0 ? ref.data() : got.data(), s->P[s->rot[0]].p, fbytes, hipMemcpyDeviceToHost));
+            float ms = 1e30f;
+            for (int pass = 0; pass < 3; ++pass) {
+                if (int rc = snap_restore(s, sn)) return rc;
+                HIP_TRY(hipEventRecord(e0, s->stream));
+                if (int rc = run_steps(s, steps_per_trial)) return rc;
+                HIP_TRY(hipEventRecord(e1, s->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float one = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&one, e0, e1));
+                ms = std::min(ms, one);
+                if (pass == 0)
+                    HIP_TRY(hipMemcpy(cand < 0 ? ref.data() : got.data(), s->P[s->rot[0]].p, fbytes,
+                                      hipMemcpyDeviceToHost));
+            }
             if (cand >= 0 && std::memcmp(ref.data(), got.data(), fbytes) != 0) continue;  // not bit-exact: rejected
             if (ms < best_ms) {
                 best_ms = ms;

@@ -87,11 +87,9 @@ struct StepArgs {
     // time
     const int64_t* d_t;         // device step counter (start of the chunk)
     int32_t t_rel;              // step offset inside the chunk
-    // fused step kernel: cells finished outside the interior tiles, and the
-    // packed coordinates (x | y << 11 | z << 22) of entries and special cells
-    const uint32_t* colmask;    // per tile slot and tile thread: bit z - z0 marks a cell finished by a list
-    const uint32_t* e_xyz;
-    const uint32_t* sp_xyz;
+    // fused step kernel: per tile slot and tile thread, bit z - z0 marks a
+    // cell the tile leaves to a list (boundary, special or PML cell)
+    const uint32_t* colmask;
     // physical PML state (compute type): phi and psi double buffers (the
     // split path uses set 0 in place), zeta at cell centres and +1/2 points,
     // and the fused path's PML cell list

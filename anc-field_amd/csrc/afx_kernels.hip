@@ -106,12 +106,6 @@ __device__ __forceinline__ C phi_update(C old, C p1j, C p1n, C pj, C pn, C psj, 
     return ((idt - (C)0.5 * h) * old + cc * g2 * (C)0.5 * (dn + dold) + cc * g3 * dps) / (idt + (C)0.5 * h);
 }
 
-__device__ __forceinline__ bool in_pml(const RoomDev& r, int x, int y, int z)
-{
-    return x < r.pml_lo[0] || x >= r.pml_hi[0] || y < r.pml_lo[1] || y >= r.pml_hi[1] || z < r.pml_lo[2] ||
-           z >= r.pml_hi[2];
-}
-
 // ---------------------------------------------------------------- interior
 // One thread owns an (x, y) column of a TX x TY x TZ tile and marches z,
 // carrying p^n at z-1, z, z+1 in registers; x and y neighbours come from the

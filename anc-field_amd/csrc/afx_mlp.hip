@@ -99,8 +99,19 @@ struct Gemm {
 struct GradProblem {
     const bf16* A;       // dY^T [M=out][lda]
     const bf16* B;       // X^T  [N=in][ldb]
-    float* out;          // dW [M][N]
+    float* out;          // dW [M][N] (without the fused update)
     int lda, ldb, M, N, K;
+    int64_t woff;        // parameter offset of W (fused update)
+    bf16* wb;            // BF16 copies W [M][N] and W^T [N][M] (fused update)
+    bf16* wt;
+};
+
+// AdamW applied where a gradient is produced (training): the gradient never
+// goes to memory.
+struct AdamArgs {
+    float *w, *m1, *m2;
+    float lr, b1, b2, eps, wd, bc1, bc2;
+    int on;              // 0: write gradients instead
 };
 
 struct GradTable {
@@ -112,6 +123,7 @@ struct GradTable {
 struct ColsumTable {
     const bf16* g[kMaxLayers];  // dY^T [rows_l][ld]
     float* out[kMaxLayers];
+    int64_t boff[kMaxLayers];   // parameter offset of b_l (fused update)
     int rows[kMaxLayers];
     int start[kMaxLayers + 1];  // first output row of each layer in the flattened range
     int ld, K, n;
@@ -162,6 +174,32 @@ __device__ inline void mma_block(const bf16* __restrict__ A, int lda, const bf16
         c[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, c[0][1], 0, 0, 0);
         c[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, c[1][0], 0, 0, 0);
         c[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c[1][1], 0, 0, 0);
+    }
+}
+
+// AdamW on parameter i with gradient gi; returns its new value.
+__device__ inline float adam_step(float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2, int64_t i,
+                                  float gi, float lr, float b1, float b2, float eps, float wd, float bc1, float bc2)
+{
+    float v = w[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float s = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = s;
+    const float mh = a / bc1, vh = s / bc2;
+    v = v - lr * (mh / (sqrtf(vh) + eps) + wd * v);
+    w[i] = v;
+    return v;
+}
+
+// Learning rate and bias corrections of the current step.
+__device__ inline void adam_sched(AdamArgs& a, const StepRef& sr)
+{
+    if (sr.sched) {
+        const float* q = sr.sched + 3 * step_of(sr);
+        a.lr = q[0];
+        a.bc1 = q[1];
+        a.bc2 = q[2];
     }
 }
 
@@ -234,7 +272,7 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
     }
 }
 
-__global__ __launch_bounds__(256) void k_gemm_grouped(GradTable t)
+__global__ __launch_bounds__(256) void k_gemm_grouped(GradTable t, AdamArgs ad, StepRef sr)
 {
     __shared__ float lds[4][4][4][64];
     int p = 0;
@@ -249,7 +287,20 @@ __global__ __launch_bounds__(256) void k_gemm_grouped(GradTable t)
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int n = n0 + 16 * (q & 1) + (lane & 15);
     const int mb = m0 + 16 * (q >> 1) + ((lane >> 4) << 2);
-    for (int e = 0; e < 4; ++e) g.out[(size_t)(mb + e) * g.N + n] = acc[e];
+    if (!ad.on) {
+        for (int e = 0; e < 4; ++e) g.out[(size_t)(mb + e) * g.N + n] = acc[e];
+        return;
+    }
+    adam_sched(ad, sr);
+    bf16x4 col;
+    for (int e = 0; e < 4; ++e) {
+        const int64_t k = (int64_t)(mb + e) * g.N + n;
+        const float v = adam_step(ad.w, ad.m1, ad.m2, g.woff + k, acc[e], ad.lr, ad.b1, ad.b2, ad.eps, ad.wd, ad.bc1,
+                                  ad.bc2);
+        col[e] = (bf16)v;
+        g.wb[k] = col[e];
+    }
+    *reinterpret_cast<bf16x4*>(g.wt + (size_t)n * g.M + mb) = col;
 }
 
 // Fourier features: thread (b, j, k) writes sin and cos of 2 pi f_k x[b][j]
@@ -278,7 +329,7 @@ __global__ void k_features(const float* __restrict__ x, const int* __restrict__ 
 }
 
 // Bias gradients: one wave per output row of the flattened layer range.
-__global__ __launch_bounds__(256) void k_colsum(ColsumTable t)
+__global__ __launch_bounds__(256) void k_colsum(ColsumTable t, AdamArgs ad, StepRef sr)
 {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -290,7 +341,13 @@ __global__ __launch_bounds__(256) void k_colsum(ColsumTable t)
     float s = 0.0f;
     for (int b = lane; b < t.K; b += 64) s += (float)g[b];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) t.out[l][n] = s;
+    if (lane != 0) return;
+    if (!ad.on) {
+        t.out[l][n] = s;
+        return;
+    }
+    adam_sched(ad, sr);
+    adam_step(ad.w, ad.m1, ad.m2, t.boff[l] + n, s, ad.lr, ad.b1, ad.b2, ad.eps, ad.wd, ad.bc1, ad.bc2);
 }
 
 // AdamW on parameter i; returns its new value (the old one when !update).
@@ -477,7 +534,8 @@ int forward_hidden(afx_mlp* s, const float* x, const int* rows, bool keep)
 }
 
 // One forward + backward over the batch in X (rows gathered from x/target/weight).
-int step_grad(afx_mlp* s, const float* x, const float* target, const float* weight, const int* rows, float* y)
+int step_grad(afx_mlp* s, const float* x, const float* target, const float* weight, const int* rows, float* y,
+              bool fused_adam = false)
 {
     int rc = forward_hidden(s, x, rows, true);
     if (rc) return rc;
@@ -536,9 +594,18 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
         p.M = s->out[l];
         p.N = s->in[l];
         p.K = s->B;
+        p.woff = s->off[l];
+        p.wb = s->wb[l];
+        p.wt = s->wt[l];
         gt.start[l + 1] = gt.start[l] + (p.M / 32) * (p.N / 32);
     }
-    k_gemm_grouped<<<gt.start[L], 256, 0, s->stream>>>(gt);
+    AdamArgs ad{};
+    if (fused_adam) {
+        s->t += 1;
+        ad = AdamArgs{s->w, s->m1, s->m2, 0.0f, s->cfg.beta1, s->cfg.beta2, s->cfg.eps, s->cfg.weight_decay,
+                      1.0f, 1.0f, 1};
+    }
+    k_gemm_grouped<<<gt.start[L], 256, 0, s->stream>>>(gt, ad, s->sr);
     HIP_CHECK(hipGetLastError());
     ColsumTable ct{};
     ct.n = L;
@@ -548,10 +615,11 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
     for (int l = 0; l < L; ++l) {
         ct.g[l] = s->DT[l];
         ct.out[l] = s->g + s->off[l] + (int64_t)s->out[l] * s->in[l];
+        ct.boff[l] = s->off[l] + (int64_t)s->out[l] * s->in[l];
         ct.rows[l] = s->out[l];
         ct.start[l + 1] = ct.start[l] + s->out[l];
     }
-    k_colsum<<<(ct.start[L] + 3) / 4, 256, 0, s->stream>>>(ct);
+    k_colsum<<<(ct.start[L] + 3) / 4, 256, 0, s->stream>>>(ct, ad, s->sr);
     HIP_CHECK(hipGetLastError());
     return AFX_OK;
 }
@@ -946,9 +1014,7 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
     s->sr = StepRef{d_step, (int64_t)B, (int64_t)s->nblk_loss, d_sched};
     // One step: forward, loss, backward, AdamW, then the device step counter.
     auto enqueue_step = [&]() -> int {
-        int rc = step_grad(s, s->xd, s->td, s->wd, d_idx, nullptr);
-        if (!rc) rc = adam(s, 0.0f, true);
-        if (rc) return rc;
+        if (int rc = step_grad(s, s->xd, s->td, s->wd, d_idx, nullptr, true)) return rc;
         k_step_inc<<<1, 64, 0, s->stream>>>(d_step);
         HIP_CHECK(hipGetLastError());
         return AFX_OK;

@@ -6,7 +6,7 @@ face, band-limited noise source, 16 receivers), FP32, physical boundary
 mode, run through the C ABI.  Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
-                  [--compat] [--no-cpu-baseline] [--tune] [--no-batched] [--no-surrogate] [--no-large]
+                  [--compat] [--no-cpu-baseline] [--no-tune] [--no-batched] [--no-surrogate] [--no-large]
 """
 import argparse
 import json
@@ -56,12 +56,12 @@ def batched_leg(steps):
            "sequential_fp32": {"ir_per_s": round(n_ir / t_seq, 2), "wall_s": round(t_seq, 3)}}
     runs = {}
     for prec in ("fp32", "fp16", "bf16", "fp64"):
-        runs[prec] = batch.run_batched(ws, steps, prec)
+        runs[prec] = batch.run_batched(ws, steps, prec, tune=True)
     ref = runs["fp64"][0]
-    for prec, (tr, t) in runs.items():
+    for prec, (tr, t, tile) in runs.items():
         res[f"batched_{prec}"] = {
             "ir_per_s": round(n_ir / t, 2), "wall_s": round(t, 3), "speedup_vs_sequential_fp32": round(t_seq / t, 2),
-            "gcell_per_s": round(cells * steps / t / 1e9, 1),
+            "gcell_per_s": round(cells * steps / t / 1e9, 1), "autotuned_tile": list(tile),
             "rel_l2_vs_fp64": batch.rel_l2(tr, ref)}
     res["batched_fp32"]["bitexact_vs_sequential"] = bool((runs["fp32"][0] == seq).all())
     return res
@@ -170,7 +170,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=sorted(BYTES_PER_CELL))
     ap.add_argument("--compat", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape (no autotuning)")
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=4096)
@@ -201,7 +201,7 @@ def main():
     w = rooms.furnished(steps=total)
     sim = rooms.make_sim(w, _afx.Sim, compat=args.compat, precision=args.precision)
     sim.run(args.warmup)
-    tile = sim.autotune(30) if args.tune else None
+    tile = None if args.no_tune else sim.autotune(30)
     sim.synchronize()
     t0 = time.perf_counter()
     sim.run(args.steps)

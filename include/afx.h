@@ -184,9 +184,9 @@ int afx_profile_run(afx_sim* sim, int64_t steps, afx_profile* out);
 int afx_set_tile(afx_sim* sim, int32_t tx, int32_t ty, int32_t tz);
 
 /* Times the candidate interior tile shapes on this handle's rooms (each is
- * first checked bit-exact against the default shape on a copy of the state),
- * keeps the fastest and returns it in tile[3].  Leaves the simulation state
- * unchanged. */
+ * first checked bit-exact against the current shape on a copy of the state;
+ * the best of three timed passes counts), keeps the fastest and returns it in
+ * tile[3].  Leaves the simulation state unchanged. */
 int afx_autotune(afx_sim* sim, int32_t steps_per_trial, int32_t tile[3]);
 
 /* Measurement aid, no counterpart in the reference: a plain device copy of

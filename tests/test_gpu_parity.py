@@ -313,3 +313,59 @@ def test_plane_snapshot_matches_field(precision):
     with pytest.raises(afx.AfxError):
         s.plane(1, 30, dims)
     s.close()
+
+
+def _workload_pair(w, steps, threads=16):
+    s = rooms.make_sim(w, afx.Sim)
+    s.run(steps)
+    got = s.receivers()
+    field = s.pressure(0)
+    s.close()
+    o = orc.from_workload(w, compat=False, threads=threads)
+    o.run(steps)
+    want, wfield = o.receivers(), o.field(0)
+    o.close()
+    return got, want, field, wfield
+
+
+def test_shoebox_small_full_run_bitexact():
+    """64x48x32 shoebox, 3 IIR materials, 8-cell PML on x+, Ricker at (10,12,8),
+    4 receivers, all 2000 steps: identical to the oracle."""
+    _case("shoebox_small 64x48x32, 2000 steps, physical PML")
+    w = rooms.shoebox_small(seed=1, steps=2000)
+    got, want, f, wf = _workload_pair(w, 2000)
+    assert np.abs(want).max() > 0
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+def test_furnished_room_bitexact():
+    """The benchmark room itself (256x192x128, 12 boxes, 4 IIR materials,
+    12-cell PML on x+, 16 receivers) for 60 steps: identical to the oracle."""
+    _case("furnished 256x192x128, 60 steps")
+    w = rooms.furnished(steps=60)
+    got, want, f, wf = _workload_pair(w, 60)
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+
+
+def test_large_room_linearity():
+    """640x480x320 (98 M cells, two PML faces): the scheme is linear, and a
+    power-of-two source scale is exact in floating point, so doubling every
+    source doubles every trace bit for bit."""
+    _case("large 640x480x320, 40 steps, source x1 and x2")
+    w = rooms.large(steps=40)
+    # receivers the wave reaches within 40 steps, next to the source
+    near = w.src_cells[0] + np.array([[2, 0, 0], [0, 3, 0], [0, 0, -4]], np.int32)
+    w.mic_cells = np.concatenate([w.mic_cells, near])
+    s1 = rooms.make_sim(w, afx.Sim)
+    s1.run(40)
+    t1 = s1.receivers()
+    s1.close()
+    w.src = 2.0 * w.src
+    s2 = rooms.make_sim(w, afx.Sim)
+    s2.run(40)
+    t2 = s2.receivers()
+    s2.close()
+    assert np.abs(t1).max() > 0
+    assert np.array_equal(t2, 2.0 * t1)
