@@ -87,3 +87,15 @@ def test_training_validation_and_fallback(data):
     assert np.array_equal(out[fb], truth[fb])
     assert np.all(np.isfinite(err[fb])) and np.all(np.isnan(err[~fb]))
     assert sur.fdtd_runs - runs == len({tuple(s) for s in qs[fb]})
+    # the gain: a validated pair costs one network row instead of an FDTD run
+    import time
+    t0 = time.perf_counter()
+    sur.predict(ps, pr)
+    t_net = (time.perf_counter() - t0) / len(ps)
+    t0, n0 = sur.fdtd_seconds, sur.fdtd_runs
+    sur.simulate(src[:8], rcv[:8])
+    t_fdtd = (sur.fdtd_seconds - t0) / (8 * 16)
+    print(f"per impulse response: network {t_net * 1e6:.1f} us, FDTD {t_fdtd * 1e6:.1f} us "
+          f"({t_fdtd / t_net:.0f}x)")
+    assert sur.fdtd_runs - n0 == 8
+    assert t_net < t_fdtd
