@@ -369,3 +369,81 @@ def test_large_room_linearity():
     s2.close()
     assert np.abs(t1).max() > 0
     assert np.array_equal(t2, 2.0 * t1)
+
+
+TILES = [(64, 4, 16), (64, 4, 32), (64, 8, 16), (64, 2, 32), (128, 2, 16), (64, 4, 8), (256, 1, 16)]
+
+
+def _pml_room():
+    v, a, b, rng = _closed_room(31, dims=(70, 40, 36))
+    v[:, :, -1] = -1   # open x+ face
+    v[:, 0, :] = -1    # open y- face
+    v = rooms.repair(v)
+    inner = rooms.interior_cells(v, rng, 3)
+    sc = np.stack([inner[0], np.array([66, 20, 15], np.int32)])   # one source in the x+ slab
+    mc = np.concatenate([inner[1:], np.array([[20, 3, 12], [66, 20, 1]], np.int32)])
+    prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
+    return v, a, b, sc, mc, prof
+
+
+@pytest.mark.parametrize("tile", TILES)
+def test_tile_shapes_bitexact(tile):
+    _case(f"tile shape {tile}, physical PML on x+ and y-")
+    v, a, b, sc, mc, prof = _pml_room()
+    steps = 120
+    dt = rooms.courant_dt(0.02)
+    src = np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 800.0, amp=0.5)])
+    s = afx.Sim(c=rooms.C_AIR, dx=0.02, dt=dt, res=0.02, compat=False)
+    s.set_voxels(v)
+    s.set_materials(a, b)
+    s.set_pml(6, 2 | 4, prof)
+    s.set_tile(*tile)
+    for cell, sig in zip(sc, src):
+        s.add_source_cell(cell, sig)
+    for cell in mc:
+        s.add_receiver_cell(cell)
+    s.run(steps)
+    got, f = s.receivers(), s.pressure(0)
+    assert (s.stats().tile_x, s.stats().tile_y, s.stats().tile_z) == tile
+    s.close()
+    o = orc.Oracle(v, a, b, rooms.C_AIR, 0.02, dt, False, sc, src, mc, "f32", 2, 6, 2 | 4, prof)
+    o.run(steps)
+    _assert_bitexact(got, o.receivers(), "traces")
+    _assert_bitexact(f, o.field(0), "field")
+
+
+def test_tile_change_mid_run_and_autotune_keep_state():
+    """A tile change between runs and an autotune pass (which steps copies of
+    the state) leave the simulation exactly where one straight run is, for a
+    batched handle with the PML on every room."""
+    _case("set_tile mid-run and autotune, batched rooms with PML")
+    ws = rooms.batch_rooms(seed=9, nrooms=3, steps=160, lo=(26, 22, 20), hi=(40, 36, 30), nmic=3)
+
+    def make():
+        s = afx.Sim(c=ws[0].c, dx=ws[0].dx, dt=ws[0].dt, res=ws[0].dx, compat=False)
+        s.set_voxels(ws[0].vox)
+        s.set_materials(ws[0].a, ws[0].b)
+        s.set_pml(4, 2 | 16)
+        ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
+        for rid, w in zip(ids, ws):
+            for cell, sig in zip(w.src_cells, w.src):
+                s.add_source_cell(cell, sig, room=rid)
+        for rid, w in zip(ids, ws):
+            for cell in w.mic_cells:
+                s.add_receiver_cell(cell, room=rid)
+        return s
+
+    s = make()
+    s.run(160)
+    want = s.receivers()
+    s.close()
+    s = make()
+    s.run(50)
+    s.set_tile(128, 2, 16)
+    s.run(50)
+    tile = s.autotune(10)
+    assert tile in TILES
+    s.run(60)
+    got = s.receivers()
+    s.close()
+    _assert_bitexact(got, want, "traces")
