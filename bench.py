@@ -131,6 +131,64 @@ def surrogate_leg(train_steps, eager=False):
         sur.close()
 
 
+def zone_leg(train_steps, n_val=32, n_test=32):
+    """Learned surrogate over a loudspeaker zone of the furnished room: a
+    9^3-cell box of candidate source cells, 16 fixed receivers.  FDTD from
+    the box's lattice of every 2nd cell (125 sources) trains an ensemble of
+    two 6 x 512 networks (fmax 16 cycles per room length); n_val held-out box
+    sources calibrate the ensemble-spread threshold (0.8 x the bound); n_test
+    others are queried.  Reported against trilinear interpolation of the same
+    lattice responses, the answer the data gives without a network."""
+    import numpy as np
+    from anc_field_py import rooms, surrogate
+    w = rooms.furnished(seed=2, steps=4096)
+    rng = np.random.Generator(np.random.PCG64(7))
+    size, step = 9, 2
+    lo = surrogate.zone_box(w, rng, size)
+    mics = surrogate.free_cells(w, rng, 16)
+    zone = surrogate.SourceZone(lo, size, step, mics)
+    sur = surrogate.Surrogate(w, seed=1, members=2, freq_min=0.25, freq_max=16.0)
+    try:
+        t = time.perf_counter()
+        loss = surrogate.fit_zone(sur, zone, train_steps, 1e-3, 1e-5, seed=3)
+        t_fit = time.perf_counter() - t
+        cells = lo + np.stack(np.meshgrid(*[np.arange(size)] * 3, indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+        off = cells[np.any((cells - lo) % step, axis=1)]
+        pick = off[rng.permutation(len(off))]
+        val, test = pick[:n_val], pick[n_val:n_val + n_test]
+        e_val, t_val, sp_val = surrogate.calibrate_zone(sur, zone, val)
+        ts, tr_ = surrogate.pairs(test, [mics] * len(test))
+        truth = sur.simulate(test, [mics] * len(test))
+        net = sur.predict(ts, tr_)
+        tri = zone.interpolate(ts, np.tile(np.arange(len(mics)), len(test)))
+        runs = sur.fdtd_runs
+        t = time.perf_counter()
+        ans, fb, _ = sur.query(ts, tr_)
+        t_query = time.perf_counter() - t
+        e_net, e_tri, e_ans = (surrogate.rel_l2(a, truth) for a in (net, tri, ans))
+        acc = ~fb
+        src_acc = int(np.sum(~fb.reshape(len(test), -1).any(axis=1)))
+        return {
+            "zone_cells": size ** 3, "lattice_sources": int(len(zone.lattice())), "receivers": int(len(mics)),
+            "members": len(sur.nets), "train_steps_per_member": int(train_steps), "fit_wall_s": round(t_fit, 1),
+            "loss_last_100": float(np.mean(loss[-100:])),
+            "validation": {"sources": int(len(val)), "ensemble_rel_l2_median": float(np.median(e_val)),
+                           "trilinear_rel_l2_median": float(np.median(t_val)),
+                           "spread_threshold": zone.delta, "margin": 0.8},
+            "test": {"sources": int(len(test)), "pairs": int(len(ts)),
+                     "ensemble_rel_l2_median": float(np.median(e_net)),
+                     "ensemble_within_bound": float(np.mean(e_net <= sur.bound)),
+                     "trilinear_rel_l2_median": float(np.median(e_tri)),
+                     "trilinear_within_bound": float(np.mean(e_tri <= sur.bound)),
+                     "sources_answered_by_net": src_acc, "fdtd_source_runs": int(sur.fdtd_runs - runs),
+                     "pairs_answered_by_net": int(acc.sum()),
+                     "max_rel_l2_of_net_answers": float(e_ans[acc].max()) if acc.any() else None,
+                     "max_rel_l2_of_all_answers": float(e_ans.max()), "query_wall_s": round(t_query, 3)},
+        }
+    finally:
+        sur.close()
+
+
 def large_leg(steps):
     """Large room, reduced-precision storage: 640 x 480 x 320 cells (98 M),
     dx 0.01 m, 6 IIR materials, 16-cell PML on x+ and y+, band-limited
@@ -189,6 +247,8 @@ def main():
     ap.add_argument("--large-steps", type=int, default=20000)
     ap.add_argument("--surrogate-steps", type=int, default=30000)
     ap.add_argument("--surrogate-eager", action="store_true", help="train without captured graphs")
+    ap.add_argument("--no-zone", action="store_true")
+    ap.add_argument("--zone-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -265,6 +325,8 @@ def main():
         out["batched"] = batched_leg(args.batched_steps)
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(args.surrogate_steps, args.surrogate_eager)
+    if not args.no_zone:
+        out["surrogate_zone"] = zone_leg(args.zone_steps)
     if not args.no_large:
         out["large"] = large_leg(args.large_steps)
     if not args.no_cpu_baseline:

@@ -81,7 +81,10 @@ def test_training_validation_and_fallback(data):
     runs = sur.fdtd_runs
     out, fb, err = sur.query(qs, qr)
     n_ok = min(24, int(ok.sum()))
-    assert not fb[:n_ok].any() and fb[n_ok:].all()
+    # a source with any untrusted pair is simulated, and its run answers all of its pairs
+    rerun = {tuple(c) for c in qs[n_ok:]}
+    assert fb.tolist() == [tuple(c) in rerun for c in qs]
+    assert not fb[:n_ok].all()
     # every answer is within the bound of the FDTD response; fallbacks are the FDTD response
     assert np.all(surrogate.rel_l2(out, truth) <= sur.bound)
     assert np.array_equal(out[fb], truth[fb])
@@ -99,3 +102,47 @@ def test_training_validation_and_fallback(data):
           f"({t_fdtd / t_net:.0f}x)")
     assert sur.fdtd_runs - n0 == 8
     assert t_net < t_fdtd
+
+
+def test_source_zone_ensemble_gain_and_gate():
+    print("\n[case] source zone: furnished 128x96x64, 9^3 box, lattice step 2, 8 receivers, 512 taps, "
+          "2 x 6000 steps", flush=True)
+    from anc_field_py import rooms, surrogate
+    taps = 512
+    w = rooms.furnished(seed=2, steps=taps * DECIM, dims=(128, 96, 64), nbox=6)
+    rng = np.random.Generator(np.random.PCG64(13))
+    size, step = 9, 2
+    lo = surrogate.zone_box(w, rng, size)
+    mics = surrogate.free_cells(w, rng, 8)
+    zone = surrogate.SourceZone(lo, size, step, mics)
+    assert len(zone.lattice()) == 125 and zone.inside(zone.lattice()).all()
+    sur = surrogate.Surrogate(w, taps=taps, decim=DECIM, members=2, seed=1, freq_min=0.25, freq_max=16.0)
+    try:
+        surrogate.fit_zone(sur, zone, 6000, 1e-3, 1e-5, seed=3)
+        cells = lo + np.stack(np.meshgrid(*[np.arange(size)] * 3, indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+        off = cells[np.any((cells - lo) % step, axis=1)]
+        pick = off[rng.permutation(len(off))]
+        val, test = pick[:16], pick[16:32]
+        e_val, t_val, sp_val = surrogate.calibrate_zone(sur, zone, val)
+        assert np.all(sp_val > 0)  # the members differ
+        ts, tr_ = surrogate.pairs(test, [mics] * len(test))
+        truth = sur.simulate(test, [mics] * len(test))
+        e_net = surrogate.rel_l2(sur.predict(ts, tr_), truth)
+        e_tri = surrogate.rel_l2(zone.interpolate(ts, np.tile(np.arange(len(mics)), len(test))), truth)
+        runs = sur.fdtd_runs
+        ans, fb, err = sur.query(ts, tr_)
+        per_src = fb.reshape(len(test), -1)
+        print(f"held-out zone sources: ensemble median rel L2 {np.median(e_net):.3e} "
+              f"(within 5e-2: {np.mean(e_net <= 0.05):.2f}), trilinear {np.median(e_tri):.3e} "
+              f"({np.mean(e_tri <= 0.05):.2f}); threshold {zone.delta:.3e}; "
+              f"sources answered by the network {int((~per_src.any(1)).sum())} of {len(test)}")
+        # the gain over the same lattice responses without a network
+        assert np.median(e_net) < 0.8 * np.median(e_tri)
+        assert np.mean(e_net <= sur.bound) >= np.mean(e_tri <= sur.bound)
+        # per-source decisions; every answer within the bound; runs only for rejected sources
+        assert np.all(per_src.all(1) | ~per_src.any(1))
+        assert np.all(surrogate.rel_l2(ans, truth) <= sur.bound)
+        assert np.array_equal(ans[fb], truth[fb])
+        assert sur.fdtd_runs - runs == int(per_src.any(1).sum())
+    finally:
+        sur.close()

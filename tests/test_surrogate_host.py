@@ -53,3 +53,35 @@ def test_nearest_pair_baseline_picks_the_nearest_training_response():
     ir = np.array([[1.0, 0.0], [0.0, 1.0]])
     got = surrogate.nearest_pair_baseline(ts, tr, ir, np.array([[9, 9, 9], [1, 0, 0]]), np.array([[19, 20, 21], [1, 2, 1]]))
     assert np.array_equal(got, ir[[1, 0]])
+
+
+def test_zone_lattice_inside_and_trilinear_interpolation():
+    z = surrogate.SourceZone([10, 20, 30], 5, 2, np.zeros((2, 3), np.int32))
+    lat = z.lattice()
+    assert len(lat) == 27 and lat[0].tolist() == [10, 20, 30] and lat[1].tolist() == [12, 20, 30]
+    assert lat[-1].tolist() == [14, 24, 34]
+    assert z.inside(np.array([[14, 24, 34], [13, 21, 30]])).all()
+    assert not z.inside(np.array([[15, 20, 30], [9, 20, 30]])).any()
+    # responses affine in the source cell (per receiver) are interpolated exactly, and
+    # lattice cells return their own response
+    coef = np.array([[0.5, -1.0, 2.0], [1.5, 0.25, -0.75]])
+    f = lambda c, r: np.array([coef[r] @ c + r, 3.0 * r - coef[r] @ c])
+    z.train_ir = np.stack([np.stack([f(c, r) for r in range(2)]) for c in lat])
+    q = np.array([[11, 23, 33], [13, 20, 31], [12, 22, 34], [14, 24, 34]])
+    for r in range(2):
+        got = z.interpolate(q, np.full(len(q), r))
+        assert np.allclose(got, [f(c, r) for c in q], rtol=0, atol=1e-4)
+
+
+def test_ensemble_spread_and_threshold():
+    a = np.array([[3.0, 4.0], [1.0, 0.0]])
+    b = np.array([[3.0, 4.0], [0.8, 0.0]])
+    sp = surrogate.Surrogate.spread(np.stack([a, b]))
+    # two members: |a - b| / sqrt(2) over the norm of their mean
+    assert np.allclose(sp, [0.0, 0.2 / np.sqrt(2) / 0.9])
+    err = np.array([0.01, 0.02, 0.06, 0.03, 0.02])
+    spread = np.array([0.1, 0.3, 0.4, 0.5, 0.35])
+    # the good answer at 0.5 lies above the bad one at 0.4, so 0.35 is the threshold
+    assert surrogate.spread_threshold(spread, err, 0.05) == 0.35
+    assert surrogate.spread_threshold(spread, err, 0.1) == 0.5
+    assert surrogate.spread_threshold(spread[2:3], err[2:3], 0.05) == -1.0
