@@ -95,6 +95,8 @@ _SIGS = {
     "afx_autotune": (C.c_int, [_P, C.c_int32, _i32p]),
     "afx_profile_run": (C.c_int, [_P, C.c_int64, C.POINTER(Profile)]),
     "afx_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "afx_set_time_block": (C.c_int, [_P, C.c_int32]),
+    "afx_get_time_block": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "afx_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -331,9 +333,23 @@ class Sim:
         _check(lib.afx_set_tile(self._h, tx, ty, tz))
 
     def autotune(self, steps_per_trial=30):
+        """Fastest tile shape (and steps per pass, see time_block) for this
+        handle's rooms; returns the tile (tx, ty, tz)."""
         t = (C.c_int32 * 3)()
         _check(lib.afx_autotune(self._h, steps_per_trial, t))
         return tuple(t)
+
+    def set_time_block(self, steps):
+        """1: one step per pass; 2: two steps per pass where tiles allow it
+        (bit-identical results)."""
+        _check(lib.afx_set_time_block(self._h, steps))
+
+    def time_block(self):
+        """(steps per pass, tiles taking two steps per pass)."""
+        k = C.c_int32()
+        n = C.c_int64()
+        _check(lib.afx_get_time_block(self._h, C.byref(k), C.byref(n)))
+        return k.value, n.value
 
 
 # ---- surrogate MLP (include/afx_mlp.h) ------------------------------------

@@ -166,18 +166,26 @@ struct afx_sim {
     std::vector<Source> srcs;
     std::vector<Mic> mics;
     int tile = 0;
+    int tblock = 1;  // steps per pass over the interior tiles (1, or 2: work kind 4)
     bool fused_ok = true;
 
     // ---- device state
     bool committed = false;
     bool dirty_special = false;
     int64_t total_cells = 0;
-    DBuf P[3];
-    int rot[3] = {0, 1, 2};
+    // Pressure fields.  rot[0] holds the newest step, rot[1] the one before,
+    // rot[2] the one before that; a single step writes into rot[2].  P[3]
+    // (rot[3]) is allocated for two steps per pass only.
+    DBuf P[4];
+    int rot[4] = {0, 1, 2, 3};
     DBuf d_rooms, d_tiles, d_colmask, d_pml_rec, d_pml_side;
+    RoomDev room0{};
     DBuf d_work, d_e_pack, d_pml_nbd;
     std::vector<std::pair<int64_t, int32_t>> by_cell;  // (cell, entry) sorted by cell
     int32_t nwork = 0;
+    DBuf d_work1, d_work2;  // the two launches of a two-step pass
+    int32_t nwork1 = 0, nwork2 = 0;
+    int64_t npair = 0;      // tiles advanced two steps per pass
     int32_t npml = 0;
     int32_t ntiles = 0;
     DBuf d_e_cell, d_e_info, d_e_mat, d_ring_in, d_ring_out, d_A, d_B;
@@ -198,7 +206,7 @@ struct afx_sim {
     int32_t box_xy_max = 0, box_z_max = 0;
     int pml_rot = 0;
     // graph
-    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;  // by buffer rotation and pass mode
     int graph_steps = 30;
     int64_t group_count[26] = {0};
     double last_run_ms = 0.0;
@@ -208,19 +216,15 @@ struct afx_sim {
 
     ~afx_sim()
     {
-        for (auto& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         for (auto e : ev_run)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void drop_graphs()
     {
-        for (auto& g : graph)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
     }
 };
 
@@ -605,27 +609,66 @@ int build_special(afx_sim* s)
         pcell.swap(pc2);
         pxyz.swap(px2);
         pflag.swap(pf2);
-        std::vector<std::vector<int4>> seq(8);
-        const int nsp = (int)cell.size();
-        for (int j = 0, q = 0; j < nsp; j += NT, ++q) seq[(size_t)(q % 8)].push_back(make_int4(3, j, std::min(j + NT, nsp), 0));
-        for (int x = 0; x < 8; ++x) {
-            auto& v = seq[(size_t)x];
-            int32_t bcur = bcount[(size_t)(x * per)], pcur = pcount[(size_t)(x * per)];
-            const int32_t bend = bcount[(size_t)((x + 1) * per)], pend = pcount[(size_t)((x + 1) * per)];
-            // a chunk goes just before the tile holding its first cell: the
-            // latency-bound chunks then overlap the streaming tiles instead
-            // of trailing them, and whichever of the two comes first brings
-            // the shared planes into L2
-            for (int64_t k = 0; k < per; ++k) {
-                const int64_t sl = k * 8 + x;
-                const int64_t rk = x * per + k;
-                for (; bcur < bcount[(size_t)rk + 1]; bcur += NT)
-                    v.push_back(make_int4(1, bcur, std::min<int32_t>(bcur + NT, bend), 0));
-                for (; pcur < pcount[(size_t)rk + 1]; pcur += NT)
-                    v.push_back(make_int4(2, pcur, std::min<int32_t>(pcur + NT, pend), 0));
-                if (table[(size_t)sl].x >= 0) v.push_back(make_int4(0, (int)sl, 0, 0));
+        // Tiles that may advance two steps in one pass (work kind 4): full
+        // tiles whose cells and one-cell halo are all interior cells of
+        // their room (inside [1, n-2] on every axis, none of them a
+        // boundary, PML or source/receiver cell).  Only computed when two
+        // steps per pass are selected.
+        std::vector<uint8_t> pair_ok(nslot, 0);
+        int64_t npair = 0;
+        if (s->tblock == 2 && ts.ty >= 2)
+            for (size_t sl = 0; sl < nslot; ++sl) {
+                const int4 t4 = table[sl];
+                if (t4.x < 0) continue;
+                const Room& r = s->rooms[(size_t)t4.x];
+                const int xa = t4.y - 1, xb = t4.y + ts.tx, ya = t4.z - 1, yb = t4.z + ts.ty, za = t4.w - 1,
+                          zb = t4.w + ts.tz;
+                if (xa < 1 || ya < 1 || za < 1 || xb > r.nx - 2 || yb > r.ny - 2 || zb > r.nz - 2) continue;
+                bool ok = true;
+                for (int z = za; z <= zb && ok; ++z)
+                    for (int y = ya; y <= yb && ok; ++y)
+                        for (int x = xa; x <= xb; ++x) {
+                            const int64_t c = r.base + ((int64_t)z * r.ny + y) * r.nx + x;
+                            if ((mask[(size_t)(c >> 5)] >> (c & 31)) & 1u) {
+                                ok = false;
+                                break;
+                            }
+                        }
+                pair_ok[sl] = ok ? 1 : 0;
+                npair += ok ? 1 : 0;
             }
-        }
+        s->npair = npair;
+        // mode 0: one step; 1: first launch of a two-step pass (eligible
+        // tiles as kind 4); 2: its second launch (eligible tiles left out)
+        auto make_seq = [&](int mode) {
+            std::vector<std::vector<int4>> seq(8);
+            const int nsp = (int)cell.size();
+            for (int j = 0, q = 0; j < nsp; j += NT, ++q)
+                seq[(size_t)(q % 8)].push_back(make_int4(3, j, std::min(j + NT, nsp), 0));
+            for (int x = 0; x < 8; ++x) {
+                auto& v = seq[(size_t)x];
+                int32_t bcur = bcount[(size_t)(x * per)], pcur = pcount[(size_t)(x * per)];
+                const int32_t bend = bcount[(size_t)((x + 1) * per)], pend = pcount[(size_t)((x + 1) * per)];
+                // a chunk goes just before the tile holding its first cell: the
+                // latency-bound chunks then overlap the streaming tiles instead
+                // of trailing them, and whichever of the two comes first brings
+                // the shared planes into L2
+                for (int64_t k = 0; k < per; ++k) {
+                    const int64_t sl = k * 8 + x;
+                    const int64_t rk = x * per + k;
+                    for (; bcur < bcount[(size_t)rk + 1]; bcur += NT)
+                        v.push_back(make_int4(1, bcur, std::min<int32_t>(bcur + NT, bend), 0));
+                    for (; pcur < pcount[(size_t)rk + 1]; pcur += NT)
+                        v.push_back(make_int4(2, pcur, std::min<int32_t>(pcur + NT, pend), 0));
+                    const int4 t4 = table[(size_t)sl];
+                    if (t4.x < 0 || (mode == 2 && pair_ok[(size_t)sl])) continue;
+                    const int kind = mode == 1 && pair_ok[(size_t)sl] ? 4 : 0;
+                    v.push_back(make_int4(kind, (int)sl, (int)pack_xyz(t4.y, t4.z, t4.w), t4.x));
+                }
+            }
+            return seq;
+        };
+        const auto seq = make_seq(0);
         // packed boundary records in work order
         std::vector<int4> epack(tb.size());
         for (size_t k = 0; k < tb.size(); ++k) {
@@ -657,13 +700,26 @@ int build_special(afx_sim* s)
         }
         HIP_TRY(upload(s->d_e_pack, epack, s->stream));
         HIP_TRY(upload(s->d_pml_nbd, nbd, s->stream));
-        size_t len = 0;
-        for (const auto& v : seq) len = std::max(len, v.size());
-        std::vector<int4> work(std::max<size_t>(8 * len, 1), make_int4(-1, 0, 0, 0));
-        for (size_t x = 0; x < 8; ++x)
-            for (size_t k = 0; k < seq[x].size(); ++k) work[8 * k + x] = seq[x][k];
-        s->nwork = (int32_t)(8 * len);
-        HIP_TRY(upload(s->d_work, work, s->stream));
+        auto interleave = [](const std::vector<std::vector<int4>>& sq) {
+            size_t len = 0;
+            for (const auto& v : sq) len = std::max(len, v.size());
+            std::vector<int4> work(std::max<size_t>(8 * len, 1), make_int4(-1, 0, 0, 0));
+            for (size_t x = 0; x < 8; ++x)
+                for (size_t k = 0; k < sq[x].size(); ++k) work[8 * k + x] = sq[x][k];
+            return work;
+        };
+        {
+            const auto work = interleave(seq);
+            s->nwork = (int32_t)work.size();
+            HIP_TRY(upload(s->d_work, work, s->stream));
+        }
+        if (s->tblock == 2) {
+            const auto w1 = interleave(make_seq(1)), w2 = interleave(make_seq(2));
+            s->nwork1 = (int32_t)w1.size();
+            s->nwork2 = (int32_t)w2.size();
+            HIP_TRY(upload(s->d_work1, w1, s->stream));
+            HIP_TRY(upload(s->d_work2, w2, s->stream));
+        }
         HIP_TRY(hipStreamSynchronize(s->stream));
     }
     // one record per PML cell: {cell, packed xyz, room << 24 | kind + 2, box}
@@ -977,6 +1033,7 @@ int commit(afx_sim* s)
             }
         rd.push_back(d);
     }
+    if (!rd.empty()) s->room0 = rd[0];
     HIP_TRY(upload(s->d_rooms, rd, s->stream));
     std::vector<int32_t> ecell((size_t)s->nb);
     std::vector<uint32_t> emat((size_t)s->nb);
@@ -1007,11 +1064,11 @@ int commit(afx_sim* s)
     }
     // fields
     const size_t fbytes = (prec_bytes(s->prec) * (size_t)s->total_cells + 3) / 4 * 4;
-    for (auto& p : s->P) {
-        HIP_TRY(p.alloc(fbytes));
-        HIP_TRY(launch_fill(p.p, (int64_t)fbytes, s->stream));
+    for (int q = 0; q < (s->tblock == 2 ? 4 : 3); ++q) {
+        HIP_TRY(s->P[q].alloc(fbytes));
+        HIP_TRY(launch_fill(s->P[q].p, (int64_t)fbytes, s->stream));
     }
-    s->rot[0] = 0; s->rot[1] = 1; s->rot[2] = 2;
+    s->rot[0] = 0; s->rot[1] = 1; s->rot[2] = 2; s->rot[3] = 3;
     HIP_TRY(s->d_t.alloc(sizeof(int64_t)));
     HIP_TRY(launch_fill(s->d_t.p, 8, s->stream));
     s->t = 0;
@@ -1062,6 +1119,7 @@ StepArgs make_args(afx_sim* s)
     a.tiles = s->d_tiles.as<int4>();
     a.ntiles = s->ntiles;
     a.rooms = s->d_rooms.as<RoomDev>();
+    a.room0 = s->room0;
     // lambda^2 in storage-arithmetic precision exactly as simulation.hpp:166
     if (s->prec == Prec::F64) {
         const double c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
@@ -1206,13 +1264,82 @@ struct Prof {
 #define PROF_END(k) \
     if (prof) HIP_TRY(prof->mark((k), s->stream, false))
 
+// Two steps per pass: the fused path with its work tables and a fourth field.
+bool pairs_active(const afx_sim* s)
+{
+    return s->tblock == 2 && use_fused(s) && !(s->cfg.flags & AFX_LIST_BLOCKS) && s->nwork1 > 0 && s->P[3].p;
+}
+
+// Buffer roles after one step, or after a two-step pass.
+void rotate_one(afx_sim* s)
+{
+    const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2];
+    s->rot[0] = r2;
+    s->rot[1] = r0;
+    s->rot[2] = r1;
+}
+void rotate_pair(afx_sim* s)
+{
+    const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2], r3 = s->rot[3];
+    s->rot[0] = r3;
+    s->rot[1] = r2;
+    s->rot[2] = r0;
+    s->rot[3] = r1;
+}
+// The rotations `k` enqueued steps make (enqueue_steps pairs steps j, j+1
+// while both are inside the k).
+void rotate_steps(afx_sim* s, int k)
+{
+    const bool pairs = pairs_active(s);
+    for (int j = 0; j < k; ++j) {
+        if (pairs && j + 1 < k) {
+            rotate_pair(s);
+            ++j;
+        } else {
+            rotate_one(s);
+        }
+    }
+}
+
 int enqueue_steps(afx_sim* s, int k, int64_t local0, Prof* prof = nullptr)
 {
+    const bool pairs = pairs_active(s);
     for (int j = 0; j < k; ++j) {
-        const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2];
-        s->rot[0] = r2;
-        s->rot[1] = r0;
-        s->rot[2] = r1;
+        if (pairs && j + 1 < k) {
+            // Launch 1: the two-step tiles write p^{n+1} into rot[2] and
+            // p^{n+2} into rot[3]; everything else takes step n+1 into
+            // rot[2].  Launch 2: everything else takes step n+2 into rot[3].
+            // Edge-plane special cells keep, in step n+2, the value a single
+            // step would have written over (p^{n-1}, rot[1]).
+            const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2], r3 = s->rot[3];
+            StepArgs a = make_args(s);
+            a.out = s->P[r2].p;
+            a.cur = s->P[r0].p;
+            a.prev = s->P[r1].p;
+            a.out2 = s->P[r3].p;
+            a.work = s->d_work1.as<int4>();
+            a.nwork = s->nwork1;
+            a.t_rel = j;
+            PROF_BEGIN(0);
+            HIP_TRY(launch_fused(a, s->stream));
+            PROF_END(0);
+            StepArgs b = a;
+            b.out = s->P[r3].p;
+            b.cur = s->P[r2].p;
+            b.prev = s->P[r0].p;
+            b.out2 = nullptr;
+            b.keep = s->P[r1].p;
+            b.work = s->d_work2.as<int4>();
+            b.nwork = s->nwork2;
+            b.t_rel = j + 1;
+            PROF_BEGIN(0);
+            HIP_TRY(launch_fused(b, s->stream));
+            PROF_END(0);
+            rotate_pair(s);
+            ++j;
+            continue;
+        }
+        rotate_one(s);
         StepArgs a = make_args(s);
         a.t_rel = j;
         if (use_fused(s)) {
@@ -1277,14 +1404,21 @@ int run_steps(afx_sim* s, int64_t steps)
     int rc = ensure_traces(s, s->t + steps);
     if (rc) return rc;
     const bool graphs = !(s->cfg.flags & AFX_NO_GRAPH) && s->pml_mode != 1;
-    const int K = s->graph_steps;  // multiple of 3: buffer roles repeat
+    const bool pairs = pairs_active(s);
+    // steps per graph: buffer roles return to the start after 30 single
+    // steps (a multiple of 3) or 16 two-step passes
+    const int K = pairs ? 32 : s->graph_steps;
     int64_t done = 0;
     while (done < steps) {
         const int64_t left = steps - done;
         if (graphs && left >= K) {
-            const int ri = s->rot[0];  // rotation phase at chunk start
-            if (!s->graph[ri]) {
-                int saved_rot[3] = {s->rot[0], s->rot[1], s->rot[2]};
+            const uint32_t key = (uint32_t)s->rot[0] | (uint32_t)s->rot[1] << 2 | (uint32_t)s->rot[2] << 4 |
+                                 (uint32_t)s->rot[3] << 6 | (pairs ? 1u << 8 : 0u);
+            hipGraphExec_t exec = nullptr;
+            for (auto& g : s->graphs)
+                if (g.first == key) exec = g.second;
+            if (!exec) {
+                int saved_rot[4] = {s->rot[0], s->rot[1], s->rot[2], s->rot[3]};
                 const int64_t saved_t = s->t;
                 hipGraph_t g = nullptr;
                 HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -1295,27 +1429,21 @@ int run_steps(afx_sim* s, int64_t steps)
                     return erc;
                 }
                 HIP_TRY(ce);
-                hipError_t ie = hipGraphInstantiate(&s->graph[ri], g, nullptr, nullptr, 0);
+                hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 HIP_TRY(ie);
+                s->graphs.push_back({key, exec});
                 // capture did not execute anything: restore the host mirrors
-                s->rot[0] = saved_rot[0];
-                s->rot[1] = saved_rot[1];
-                s->rot[2] = saved_rot[2];
+                std::copy(saved_rot, saved_rot + 4, s->rot);
                 s->t = saved_t;
             }
-            HIP_TRY(hipGraphLaunch(s->graph[ri], s->stream));
+            HIP_TRY(hipGraphLaunch(exec, s->stream));
             // advance host mirrors as the graph did
-            for (int j = 0; j < K; ++j) {
-                const int r0 = s->rot[0], r1 = s->rot[1], r2 = s->rot[2];
-                s->rot[0] = r2;
-                s->rot[1] = r0;
-                s->rot[2] = r1;
-            }
+            rotate_steps(s, K);
             s->t += K;
             done += K;
         } else {
-            const int k = (int)std::min<int64_t>(left, graphs ? (int64_t)1 : (int64_t)64);
+            const int k = (int)std::min<int64_t>(left, graphs ? (pairs ? (int64_t)2 : (int64_t)1) : (int64_t)64);
             rc = enqueue_steps(s, k, done);
             if (rc) return rc;
             done += k;
@@ -1875,14 +2003,14 @@ namespace {
 struct Snapshot {
     std::vector<std::unique_ptr<DBuf>> bufs;
     std::vector<const DBuf*> src;
-    int rot[3];
+    int rot[4];
     int64_t t;
     int pml_rot;
 };
 
 int snap_take(afx_sim* s, Snapshot& sn)
 {
-    std::vector<const DBuf*> list = {&s->P[0], &s->P[1], &s->P[2], &s->d_ring_in, &s->d_ring_out, &s->d_traces, &s->d_t,
+    std::vector<const DBuf*> list = {&s->P[0], &s->P[1], &s->P[2], &s->P[3], &s->d_ring_in, &s->d_ring_out, &s->d_traces, &s->d_t,
                                      &s->d_psi, &s->d_psi_old};
     for (int d = 0; d < 3; ++d) {
         list.push_back(&s->d_phi[d]);
@@ -1896,7 +2024,7 @@ int snap_take(afx_sim* s, Snapshot& sn)
         sn.bufs.push_back(std::move(c));
         sn.src.push_back(b);
     }
-    std::copy(s->rot, s->rot + 3, sn.rot);
+    std::copy(s->rot, s->rot + 4, sn.rot);
     sn.t = s->t;
     sn.pml_rot = s->pml_rot;
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1907,20 +2035,34 @@ int snap_restore(afx_sim* s, const Snapshot& sn)
 {
     for (size_t i = 0; i < sn.bufs.size(); ++i)
         HIP_TRY(hipMemcpyAsync(sn.src[i]->p, sn.bufs[i]->p, sn.bufs[i]->bytes, hipMemcpyDeviceToDevice, s->stream));
-    std::copy(sn.rot, sn.rot + 3, s->rot);
+    std::copy(sn.rot, sn.rot + 4, s->rot);
     s->t = sn.t;
     s->pml_rot = sn.pml_rot;
     HIP_TRY(hipStreamSynchronize(s->stream));
     return AFX_OK;
 }
 
-int set_tile(afx_sim* s, int tile)
+// Tile shape and steps per pass of a committed handle: the fourth field is
+// allocated (zeroed) the first time two steps per pass are chosen.
+int set_tile_tb(afx_sim* s, int tile, int tb)
 {
     s->tile = tile;
+    s->tblock = tb;
     s->drop_graphs();
+    if (tb == 2 && !s->P[3].p) {
+        HIP_TRY(s->P[3].alloc(s->P[0].bytes));
+        HIP_TRY(launch_fill(s->P[3].p, (int64_t)s->P[0].bytes, s->stream));
+    }
     int rc = build_tile_tables(s);
     if (rc) return rc;
     return build_special(s);
+}
+
+int set_tile(afx_sim* s, int tile) { return set_tile_tb(s, tile, s->tblock); }
+
+bool can_pair(const afx_sim* s, int tile)
+{
+    return use_fused(s) && !(s->cfg.flags & AFX_LIST_BLOCKS) && kTiles[tile].ty >= 2;
 }
 
 struct TuneKey {
@@ -1931,7 +2073,7 @@ struct TuneKey {
         return std::tie(dims, prec, compat, pml) < std::tie(o.dims, o.prec, o.compat, o.pml);
     }
 };
-std::map<TuneKey, int> g_tune_cache;
+std::map<TuneKey, std::pair<int, int>> g_tune_cache;  // -> (tile, steps per pass)
 
 }  // namespace
 
@@ -1949,6 +2091,27 @@ extern "C" int afx_set_tile(afx_sim* s, int32_t tx, int32_t ty, int32_t tz)
             return set_tile(s, k);
         }
     return fail(AFX_E_ARG, "tile %dx%dx%d is not one of the compiled shapes", tx, ty, tz);
+}
+
+extern "C" int afx_set_time_block(afx_sim* s, int32_t steps)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (steps != 1 && steps != 2) return fail(AFX_E_ARG, "steps per pass must be 1 or 2, not %d", steps);
+    if (!s->committed) {
+        s->tblock = steps;
+        return AFX_OK;
+    }
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return set_tile_tb(s, s->tile, steps);
+}
+
+extern "C" int afx_get_time_block(afx_sim* s, int32_t* steps, int64_t* pair_tiles)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (steps) *steps = s->tblock;
+    if (pair_tiles) *pair_tiles = s->committed && pairs_active(s) ? s->npair : 0;
+    return AFX_OK;
 }
 
 extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3])
@@ -1969,24 +2132,29 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
     key.pml = s->pml_mode;
     auto hit = g_tune_cache.find(key);
     if (hit != g_tune_cache.end()) {
-        if (hit->second != s->tile)
-            if (int rc = set_tile(s, hit->second)) return rc;
+        if (hit->second.first != s->tile || hit->second.second != s->tblock)
+            if (int rc = set_tile_tb(s, hit->second.first, hit->second.second)) return rc;
     } else {
         Snapshot sn;
         if (int rc = snap_take(s, sn)) return rc;
-        const int base_tile = s->tile;
+        // candidates: every tile shape, with one and (where the fused path
+        // allows it) two steps per pass; the current setting first, as the
+        // reference of the bit-exactness check
+        std::vector<std::pair<int, int>> cands = {{s->tile, s->tblock}};
+        for (int tl = 0; tl < kNumTiles; ++tl)
+            for (int tb = 1; tb <= (can_pair(s, tl) ? 2 : 1); ++tb)
+                if (tl != s->tile || tb != s->tblock) cands.push_back({tl, tb});
         const size_t fbytes = s->P[0].bytes;
         std::vector<char> ref(fbytes), got(fbytes);
-        int best = base_tile;
+        std::pair<int, int> best = cands[0];
         float best_ms = 1e30f;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        for (int cand = -1; cand < kNumTiles; ++cand) {
-            const int tl = cand < 0 ? base_tile : cand;
-            if (cand == base_tile) continue;
+        for (size_t ci = 0; ci < cands.size(); ++ci) {
+            const bool is_ref = ci == 0;
             if (int rc = snap_restore(s, sn)) return rc;
-            if (int rc = set_tile(s, tl)) return rc;
+            if (int rc = set_tile_tb(s, cands[ci].first, cands[ci].second)) return rc;
             // untimed pass (captures graphs, warms caches), then three timed
             // passes from the same state; the first is also the check pass
             if (int rc = run_steps(s, steps_per_trial)) return rc;
@@ -2001,19 +2169,19 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
                 HIP_TRY(hipEventElapsedTime(&one, e0, e1));
                 ms = std::min(ms, one);
                 if (pass == 0)
-                    HIP_TRY(hipMemcpy(cand < 0 ? ref.data() : got.data(), s->P[s->rot[0]].p, fbytes,
+                    HIP_TRY(hipMemcpy(is_ref ? ref.data() : got.data(), s->P[s->rot[0]].p, fbytes,
                                       hipMemcpyDeviceToHost));
             }
-            if (cand >= 0 && std::memcmp(ref.data(), got.data(), fbytes) != 0) continue;  // not bit-exact: rejected
+            if (!is_ref && std::memcmp(ref.data(), got.data(), fbytes) != 0) continue;  // not bit-exact: rejected
             if (ms < best_ms) {
                 best_ms = ms;
-                best = tl;
+                best = cands[ci];
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (int rc = snap_restore(s, sn)) return rc;
-        if (int rc = set_tile(s, best)) return rc;
+        if (int rc = set_tile_tb(s, best.first, best.second)) return rc;
         g_tune_cache[key] = best;
     }
     if (tile) {

@@ -52,9 +52,12 @@ struct StepArgs {
     void* out;               // p^{n+1}
     const void* cur;         // p^n
     const void* prev;        // p^{n-1}
+    void* out2;              // p^{n+2} of two-step tiles (work kind 4), else null
+    const void* keep;        // buffer edge-plane special cells keep (null: out)
     const int4* tiles;       // {room, x0, y0, z0}; room < 0 marks padding
     int32_t ntiles;
     const RoomDev* rooms;
+    RoomDev room0;           // rooms[0], by value: tiles of room 0 need no load
     // scalars (double so one struct serves FP32 and FP64 kernels)
     double l2, c2;           // interior lambda^2 and 2(1-3 lambda^2)
     double lam_b, l2b, c2b;  // boundary 'l', l*l, 2(1-3 l*l)

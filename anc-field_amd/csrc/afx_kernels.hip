@@ -172,6 +172,123 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
     }
 }
 
+// Two steps of one tile whose cells and one-cell halo are all interior cells
+// (none is a boundary, PML or source/receiver cell or lies on the grid edge).
+// Step n+1 runs on the tile and its halo, z-plane by z-plane, from planes of
+// p^n staged in LDS (a ring of four, each covering the tile plus two cells);
+// step n+2 runs on the tile from the planes of p^{n+1} left in LDS (a ring of
+// three).  p^{n+1} of the tile goes to out, p^{n+2} to out2.  Arithmetic and
+// sum order are interior_column's, and p^{n+1} is rounded to the storage
+// type before step n+2 reads it, as a stored value would be, so both steps
+// are bit-identical to two single steps.  HBM traffic: p^n and p^{n-1} read
+// once, two planes written: 16 B per two FP32 cell updates instead of 24.
+template <int TX, int TY> struct PairLds {
+    static constexpr int EX = TX + 2, EY = TY + 2, RX = TX + 4, RY = TY + 4;
+    static constexpr int NE = EX * EY, NR = RX * RY;
+    static constexpr int size = TY >= 2 ? 4 * NR + 3 * NE : 0;  // elements; no pair path for one-row tiles
+};
+
+template <typename S, typename C, int TX, int TY, int TZ>
+__device__ __forceinline__ void interior_pair(S* __restrict__ o1, S* __restrict__ o2, const S* __restrict__ cur,
+                                              const S* __restrict__ prev, const RoomDev& r, int x0, int y0, int z0,
+                                              C l2, C c2, C* __restrict__ lds)
+{
+    using L = PairLds<TX, TY>;
+    constexpr int NT = TX * TY;
+    constexpr int EX = L::EX, RX = L::RX, NE = L::NE, NR = L::NR;
+    constexpr int KE = (NE + NT - 1) / NT, KR = (NR + NT - 1) / NT;
+    C* sp = lds;           // [4][NR]: p^n, plane z in slot z & 3
+    C* sq = lds + 4 * NR;  // [3][NE]: p^{n+1}, plane z in slot z % 3
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    const int64_t rb = r.base + (int64_t)(y0 - 2) * sy + (x0 - 2);  // plane origin of the p^n rectangle
+    const int64_t eb = r.base + (int64_t)(y0 - 1) * sy + (x0 - 1);  // plane origin of tile + halo
+    const int tid = (int)threadIdx.x;
+    int roff[KR], eoff[KE], erc[KE];
+    int64_t rg[KR], eg[KE];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {  // rectangle cells of this thread (the last ones clamp and are not stored)
+        const int c = tid + k * NT;
+        const int cc = c < NR ? c : NR - 1;
+        roff[k] = c < NR ? c : -1;
+        rg[k] = rb + (int64_t)(cc / RX) * sy + (cc % RX);
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {  // tile + halo cells of this thread
+        const int c = tid + k * NT;
+        const int cc = c < NE ? c : NE - 1;
+        eoff[k] = c < NE ? c : -1;
+        const int ex = cc % EX, ey = cc / EX;
+        eg[k] = eb + (int64_t)ey * sy + ex;
+        erc[k] = (ey + 1) * RX + (ex + 1);
+    }
+    const int tx = tid % TX, ty = tid / TX;
+    const int ec = (ty + 1) * EX + (tx + 1);  // own cell in a p^{n+1} plane
+    const int rc = (ty + 2) * RX + (tx + 2);  // own cell in a p^n plane
+    const int64_t og = r.base + (int64_t)(y0 + ty) * sy + (x0 + tx);
+    // p^n planes z0-2 and z0-1 to LDS; plane z0 of p^n and plane z0-1 of
+    // p^{n-1} into registers (each loop pass prefetches one plane ahead)
+    C rp[KR], rm[KE];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        if (roff[k] >= 0) {
+            sp[((z0 - 2) & 3) * NR + roff[k]] = ld<S, C>(cur, (int64_t)(z0 - 2) * sz + rg[k]);
+            sp[((z0 - 1) & 3) * NR + roff[k]] = ld<S, C>(cur, (int64_t)(z0 - 1) * sz + rg[k]);
+        }
+        rp[k] = ld<S, C>(cur, (int64_t)z0 * sz + rg[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) rm[k] = ld<S, C>(prev, (int64_t)(z0 - 1) * sz + eg[k]);
+    for (int z = z0 - 1; z <= z0 + TZ; ++z) {
+        // p^n plane z+1 to LDS; prefetch p^n plane z+2 and p^{n-1} plane z+1
+        // (clamped to the last planes the tile reads)
+        C pm[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) pm[k] = rm[k];
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+            if (roff[k] >= 0) sp[((z + 1) & 3) * NR + roff[k]] = rp[k];
+        const int zn = z + 2 <= z0 + TZ + 1 ? z + 2 : z0 + TZ + 1;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) rp[k] = ld<S, C>(cur, (int64_t)zn * sz + rg[k]);
+        const int zm = z + 1 <= z0 + TZ ? z + 1 : z0 + TZ;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) rm[k] = ld<S, C>(prev, (int64_t)zm * sz + eg[k]);
+        __syncthreads();
+        // step n+1 at plane z over tile + halo
+        const C* P0 = sp + (z & 3) * NR;
+        const C* Pu = sp + ((z + 1) & 3) * NR;
+        const C* Pd = sp + ((z - 1) & 3) * NR;
+        C* Q = sq + (z % 3) * NE;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            if (eoff[k] < 0) continue;
+            const int e = erc[k];
+            C s = P0[e + 1] + P0[e - 1];
+            s = s + P0[e + RX];
+            s = s + P0[e - RX];
+            s = s + Pu[e];
+            s = s + Pd[e];
+            const C v = l2 * s + c2 * P0[e];
+            Q[eoff[k]] = rt<S, C>(v - pm[k]);
+        }
+        __syncthreads();
+        if (z >= z0 && z < z0 + TZ) st<S, C>(o1, (int64_t)z * sz + og, Q[ec]);
+        // step n+2 at plane z-1 over the tile
+        if (z - 1 >= z0) {
+            const int zc = z - 1;
+            const C* Q0 = sq + (zc % 3) * NE;
+            const C* Qd = sq + ((zc + 2) % 3) * NE;
+            C s = Q0[ec + 1] + Q0[ec - 1];
+            s = s + Q0[ec + EX];
+            s = s + Q0[ec - EX];
+            s = s + Q[ec];
+            s = s + Qd[ec];
+            const C v = l2 * s + c2 * Q0[ec];
+            st<S, C>(o2, (int64_t)zc * sz + og, v - sp[(zc & 3) * NR + rc]);
+        }
+    }
+}
+
 template <typename S, int TX, int TY, int TZ>
 __global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const S* __restrict__ cur,
                                                       const S* __restrict__ prev, const int4* __restrict__ tiles,
@@ -307,7 +424,10 @@ struct BndCtx {
     S* out;
     const S* cur;
     const S* prev;
+    S* out2;          // p^{n+2} of two-step tiles (work kind 4)
+    const S* keep;    // the buffer whose value edge-plane special cells keep
     const RoomDev* rooms;
+    RoomDev room0;
     const int32_t* e_cell;
     const uint32_t* e_info;
     const uint32_t* e_mat;
@@ -425,8 +545,10 @@ __device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsi
         const RoomDev r = c.rooms[c.sp_room[j]];
         v = rt<S, C>(leapfrog_value<S, C>(c.cur, c.prev, i, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k.l2, c.k.c2));
     } else {
-        // edge planes (b == -2) and add-only mode keep the stored value
-        v = ld<S, C>(c.out, i);
+        // edge planes (b == -2) and add-only mode keep the stored value: the
+        // output buffer's, or with two steps per pass, for the second step,
+        // the buffer a single step would have written into
+        v = ld<S, C>(c.keep, i);
     }
     finish_special<S, C>(c, j, i, t, v);
 }
@@ -602,7 +724,10 @@ __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
     c.out = (S*)a.out;
     c.cur = (const S*)a.cur;
     c.prev = (const S*)a.prev;
+    c.out2 = (S*)a.out2;
+    c.keep = a.keep ? (const S*)a.keep : (const S*)a.out;
     c.rooms = a.rooms;
+    c.room0 = a.room0;
     c.e_cell = a.e_cell;
     c.e_info = a.e_info;
     c.e_mat = a.e_mat;
@@ -694,26 +819,38 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
 {
     using C = typename Compute<S>::T;
     constexpr int NT = TX * TY;
-    __shared__ C pml_lds[3][NT];
-    // Work table: item (kind, a, b).  kind 0: interior tile slot a;
-    // 1: packed boundary entries e_pack[a..b); 2: PML list [a..b);
-    // 3: special cells [a..b); -1: idle.  Items of one XCD (blockIdx % 8)
-    // follow its tiles in order, each list chunk right after the tile holding
-    // its last cell, so both read the same planes of p^n through L2.
+    constexpr int kLds = PairLds<TX, TY>::size > 3 * NT ? PairLds<TX, TY>::size : 3 * NT;
+    __shared__ C lds[kLds];  // PML chunks: phi^n by thread; two-step tiles: pressure planes
+    // Work table: item (kind, a, b, w).  kind 0: interior tile of slot a,
+    // origin packed in b (x | y << 11 | z << 22), room w; 4: the same tile
+    // advanced two steps (interior_pair); 1: packed boundary entries
+    // e_pack[a..b); 2: PML list [a..b); 3: special cells [a..b); -1: idle.
+    // Items of one XCD (blockIdx % 8) follow its tiles in order, each list
+    // chunk right before the tile holding its first cell, so both read the
+    // same planes of p^n through L2.
     const int4 wk = work[blockIdx.x];
+    if constexpr (PairLds<TX, TY>::size > 0) {
+        if (wk.x == 4) {
+            const RoomDev r = wk.w == 0 ? c.room0 : c.rooms[wk.w];
+            const uint32_t o = (uint32_t)wk.z;
+            interior_pair<S, C, TX, TY, TZ>(c.out, c.out2, c.cur, c.prev, r, (int)(o & 2047u),
+                                            (int)((o >> 11) & 2047u), (int)(o >> 22), c.k.l2, c.k.c2, lds);
+            return;
+        }
+    }
     if (wk.x == 0) {
-        const int4 t4 = tiles[wk.y];
-        const RoomDev r = c.rooms[t4.x];
-        const int x = t4.y + (int)(threadIdx.x % TX);
-        const int y = t4.z + (int)(threadIdx.x / TX);
+        const RoomDev r = wk.w == 0 ? c.room0 : c.rooms[wk.w];
+        const uint32_t o = (uint32_t)wk.z;
+        const int x = (int)(o & 2047u) + (int)(threadIdx.x % TX);
+        const int y = (int)((o >> 11) & 2047u) + (int)(threadIdx.x / TX);
         const uint32_t bits = colmask[(int64_t)wk.y * NT + threadIdx.x];
-        interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, bits);
+        interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, (int)(o >> 22), c.k.l2, c.k.c2, bits);
         return;
     }
     if (wk.x < 0) return;
     const int64_t t = *d_t + t_rel;
     if (wk.x == 2) {  // whole workgroup: it meets a barrier
-        pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, pml_lds);
+        pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, reinterpret_cast<C(*)[NT]>(lds));
         return;
     }
     const int q = wk.y + (int)threadIdx.x;

@@ -19,7 +19,8 @@
  *   afx_get_plane                        vis cross section copy_plane.cl:29-34, simulation.hpp:286-289,394-401
  *   afx_add_room, afx_*_cell             (no counterpart: batched rooms and cell-addressed setup)
  *   afx_get_stats                        (boundary counts logged at lrs-data.hpp:212-214)
- *   afx_autotune                         (no counterpart: tile-shape search)
+ *   afx_autotune                         (no counterpart: tile-shape and time-block search)
+ *   afx_set_time_block, afx_get_time_block (no counterpart: two steps per pass)
  *   afx_profile_run                      (no counterpart: per-kernel event timing)
  *   afx_synchronize                      clh Device::queue().finish(), simulation.hpp:407
  *
@@ -183,10 +184,27 @@ int afx_profile_run(afx_sim* sim, int64_t steps, afx_profile* out);
  * 256x1x16.  Results do not depend on the shape. */
 int afx_set_tile(afx_sim* sim, int32_t tx, int32_t ty, int32_t tz);
 
-/* Times the candidate interior tile shapes on this handle's rooms (each is
- * first checked bit-exact against the current shape on a copy of the state;
- * the best of three timed passes counts), keeps the fastest and returns it in
- * tile[3].  Leaves the simulation state unchanged. */
+/* Time-step blocking.  With steps = 2 each pass over the grid advances two
+ * steps: every interior tile whose cells and one-cell halo are all plain
+ * interior cells (no boundary, PML, source or receiver cell, none on the grid
+ * edge) computes step n+1 on itself and its halo and step n+2 on itself in
+ * one launch, reading p^n and p^{n-1} once (16 instead of 24 bytes of HBM
+ * traffic per two FP32 cell updates); all other cells take the two steps in
+ * two launches.  Results are bit-identical to steps = 1.  It takes a fourth
+ * pressure field and applies to the fused step path (not with AFX_SPLIT,
+ * AFX_LIST_BLOCKS, the compat PML, or one-row tile shapes, where steps stay
+ * single).  Default 1. */
+int afx_set_time_block(afx_sim* sim, int32_t steps);
+/* The steps per pass set, and (pair_tiles, may be NULL) how many tiles take
+ * two steps per pass on the committed handle (0 when single steps run). */
+int afx_get_time_block(afx_sim* sim, int32_t* steps, int64_t* pair_tiles);
+
+/* Times the candidate interior tile shapes, each with one and two steps per
+ * pass where that applies, on this handle's rooms (each is first checked
+ * bit-exact against the current setting on a copy of the state; the best of
+ * three timed passes counts), keeps the fastest and returns its tile in
+ * tile[3] (afx_get_time_block gives its steps per pass).  Leaves the
+ * simulation state unchanged. */
 int afx_autotune(afx_sim* sim, int32_t steps_per_trial, int32_t tile[3]);
 
 /* Measurement aid, no counterpart in the reference: a plain device copy of

@@ -1,0 +1,167 @@
+"""Two steps per pass (time-step blocking) against the CPU oracle and against
+one step per pass, through the C ABI.
+
+A tile takes two steps in one pass only when its cells and one-cell halo are
+all interior cells; every other cell takes them in two launches, and four
+pressure fields rotate.  Everything must stay bit-identical to single steps:
+odd step counts (a single step after the pairs), graph chunks, a source and
+receiver on the grid's edge plane (whose stored value is carried from the
+buffer a single step would write into), the physical PML, batched rooms,
+every storage precision, plane snapshots, and switching mid-run.
+"""
+import numpy as np
+import pytest
+
+from anc_field_py import rooms
+import orc
+
+pytestmark = pytest.mark.gpu
+
+afx = pytest.importorskip("anc_field_py._afx")
+
+PAIR_TILES = [(64, 4, 16), (64, 4, 32), (64, 8, 16), (64, 2, 32), (128, 2, 16), (64, 4, 8)]
+
+
+def _case(name):
+    print(f"\n[case] {name}", flush=True)
+
+
+def _assert_bitexact(got, want, what):
+    got = np.asarray(got)
+    want = np.asarray(want).astype(got.dtype)
+    if not np.array_equal(got, want):
+        d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+        raise AssertionError(f"{what}: {np.count_nonzero(d)} values differ, max |d| {d.max():.3e}")
+
+
+def _room():
+    """264x40x70 shoebox, three boxes, open x+ face with a 6-cell PML: large
+    enough that every tile shape has tiles whose halo avoids walls and PML."""
+    rng = np.random.Generator(np.random.PCG64(41))
+    nx, ny, nz = 264, 40, 70
+    a, b = rooms.materials(rng, 3)
+    v = rooms.shoebox(nx, ny, nz, [int(rng.integers(0, 3)) for _ in range(6)])
+    v = rooms.add_boxes(v, rng, 3, 4, 9, 3)
+    v[:, :, -1] = -1
+    v = rooms.repair(v)
+    inner = rooms.interior_cells(v, rng, 4)
+    plane = np.array([0, 5, 5], np.int32)  # on the x = 0 edge plane
+    sc = np.stack([inner[0], plane])
+    mc = np.stack([inner[1], inner[2], inner[3], plane])
+    prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
+    return v, a, b, sc, mc, prof
+
+
+def _sim(v, a, b, sc, mc, prof, src, tile=None, tblock=1, precision="fp32", **kw):
+    dt = rooms.courant_dt(0.02)
+    s = afx.Sim(c=rooms.C_AIR, dx=0.02, dt=dt, res=0.02, compat=False, precision=precision, **kw)
+    s.set_voxels(v)
+    s.set_materials(a, b)
+    s.set_pml(6, 2, prof)
+    if tile:
+        s.set_tile(*tile)
+    s.set_time_block(tblock)
+    for cell, sig in zip(sc, src):
+        s.add_source_cell(cell, sig)
+    for cell in mc:
+        s.add_receiver_cell(cell)
+    return s
+
+
+def _src(steps):
+    dt = rooms.courant_dt(0.02)
+    return np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 800.0, amp=0.5)])
+
+
+@pytest.mark.parametrize("tile", PAIR_TILES)
+def test_two_steps_per_pass_match_oracle(tile):
+    _case(f"two steps per pass, tile {tile}, 121 steps, PML x+, edge-plane source")
+    v, a, b, sc, mc, prof = _room()
+    steps = 121
+    src = _src(steps)
+    s = _sim(v, a, b, sc, mc, prof, src, tile=tile, tblock=2)
+    s.run(steps)
+    k, npair = s.time_block()
+    got, f = s.receivers(), s.pressure(0)
+    s.close()
+    print(f"tiles taking two steps per pass: {npair}")
+    assert k == 2 and npair > 0
+    dt = rooms.courant_dt(0.02)
+    o = orc.Oracle(v, a, b, rooms.C_AIR, 0.02, dt, False, sc, src, mc, "f32", 2, 6, 2, prof)
+    o.run(steps)
+    _assert_bitexact(got, o.receivers(), "traces")
+    _assert_bitexact(f, o.field(0), "field")
+    assert np.abs(got).max() > 0 and np.count_nonzero(f) > 10000  # the waves spread over many tiles
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp16", "bf16"])
+def test_two_steps_per_pass_every_precision(precision):
+    _case(f"two vs one step per pass, {precision}, 75 steps, no graphs and graphs")
+    v, a, b, sc, mc, prof = _room()
+    steps = 75
+    src = _src(steps)
+    out = []
+    for tblock, graph in ((1, True), (2, True), (2, False)):
+        s = _sim(v, a, b, sc, mc, prof, src, tblock=tblock, precision=precision, graph=graph)
+        s.run(steps)
+        out.append((s.receivers(), [s.plane(2, 20, v.shape, which=w) for w in range(3)], s.time_block()[1]))
+        s.close()
+    (t1, p1, _), (t2, p2, n2), (t3, p3, n3) = out
+    assert n2 > 0 and n3 > 0
+    _assert_bitexact(t2, t1, "traces, graphs")
+    _assert_bitexact(t3, t1, "traces, no graphs")
+    for w in range(3):  # newest, previous and the one before: same roles
+        _assert_bitexact(p2[w], p1[w], f"plane which={w}, graphs")
+        _assert_bitexact(p3[w], p1[w], f"plane which={w}, no graphs")
+
+
+def test_switching_mid_run_batched_with_autotune():
+    """Batched rooms with the PML on every room: single steps, then two steps
+    per pass from an odd step, then an autotune pass, then more steps, equal
+    one straight single-step run."""
+    _case("time block switch mid-run and autotune, batched rooms with PML")
+    ws = rooms.batch_rooms(seed=12, nrooms=3, steps=150, lo=(140, 30, 40), hi=(160, 40, 50), nmic=3)
+
+    def make():
+        s = afx.Sim(c=ws[0].c, dx=ws[0].dx, dt=ws[0].dt, res=ws[0].dx, compat=False)
+        s.set_voxels(ws[0].vox)
+        s.set_materials(ws[0].a, ws[0].b)
+        s.set_pml(4, 2 | 16)
+        ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
+        for rid, w in zip(ids, ws):
+            for cell, sig in zip(w.src_cells, w.src):
+                s.add_source_cell(cell, sig, room=rid)
+        for rid, w in zip(ids, ws):
+            for cell in w.mic_cells:
+                s.add_receiver_cell(cell, room=rid)
+        return s
+
+    s = make()
+    s.run(150)
+    want = s.receivers()
+    s.close()
+    s = make()
+    s.run(37)
+    s.set_time_block(2)
+    s.run(41)
+    assert s.time_block()[1] > 0
+    tile = s.autotune(10)
+    k, _ = s.time_block()
+    print(f"autotuned: tile {tile}, {k} step(s) per pass")
+    s.run(40)
+    s.set_time_block(1)
+    s.run(32)
+    got = s.receivers()
+    s.close()
+    _assert_bitexact(got, want, "traces")
+
+
+def test_time_block_argument_checks():
+    _case("time block argument checks")
+    v, a, b, sc, mc, prof = _room()
+    s = _sim(v, a, b, sc, mc, prof, _src(4), tile=(256, 1, 16), tblock=2)
+    s.run(4)
+    assert s.time_block() == (2, 0)  # one-row tiles: single steps
+    with pytest.raises(afx.AfxError):
+        s.set_time_block(3)
+    s.close()
