@@ -31,14 +31,15 @@ def make_batch(workloads, precision="fp32", compat=False, **kw):
 
 def run_batched(workloads, steps, precision="fp32", tune=False, **kw):
     """Returns (traces [sum of receivers, steps], wall seconds incl. setup,
-    tile shape).  With tune, afx_autotune picks the tile shape first (every
-    candidate checked bit-exact); tuning time is not included."""
+    tile shape).  With tune, afx_autotune picks the tile shape and the steps
+    per pass first (every candidate checked bit-exact; the tile returned is
+    then (tx, ty, tz, steps per pass)); tuning time is not included."""
     t0 = time.perf_counter()
     s = make_batch(workloads, precision, **kw)
     tile = None
     if tune:
         t1 = time.perf_counter()
-        tile = s.autotune(30)
+        tile = s.autotune(30) + (s.time_block()[0],)
         t0 += time.perf_counter() - t1
     s.run(steps)
     tr = s.receivers()

@@ -69,7 +69,7 @@ def batched_leg(steps):
         res[f"batched_{prec}"] = {
             "ir_per_s": round(n_ir / t, 2), "wall_s": round(t, 3), "speedup_vs_sequential_fp32": round(t_seq / t, 2),
             "speedup_vs_sequential_split_eager": round(t_split / t, 2),
-            "gcell_per_s": round(cells * steps / t / 1e9, 1), "autotuned_tile": list(tile),
+            "gcell_per_s": round(cells * steps / t / 1e9, 1), "autotuned_tile": list(tile[:3]), "steps_per_pass": int(tile[3]),
             "rel_l2_vs_fp64": batch.rel_l2(tr, ref)}
     res["batched_fp32"]["bitexact_vs_sequential"] = bool((runs["fp32"][0] == seq).all())
     return res
@@ -189,19 +189,24 @@ def zone_leg(train_steps, n_val=32, n_test=32):
         sur.close()
 
 
-def large_leg(steps):
+def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps_per_pass")):
     """Large room, reduced-precision storage: 640 x 480 x 320 cells (98 M),
     dx 0.01 m, 6 IIR materials, 16-cell PML on x+ and y+, band-limited
     noise, 32 receivers.  The same run in FP64, FP32, FP16 and BF16 storage;
-    receiver-trace error of each against FP64."""
+    receiver-trace error of each against FP64.  FP32 once more with two steps
+    per pass (time-step blocking), whose traces must equal FP32's bit for
+    bit."""
     import numpy as np
     from anc_field_py import _afx, rooms
     w = rooms.large(steps=steps)
     res = {"cells": w.cells, "steps": steps, "receivers": int(len(w.mic_cells))}
     traces = {}
-    for prec in ("fp64", "fp32", "fp16", "bf16"):
-        s = rooms.make_sim(w, _afx.Sim, precision=prec)
+    for prec in precisions:
+        two = prec.endswith("two_steps_per_pass")
+        s = rooms.make_sim(w, _afx.Sim, precision="fp32" if two else prec)
         try:
+            if two:
+                s.set_time_block(2)
             s.run(1)
             s.synchronize()
             t0 = time.perf_counter()
@@ -209,12 +214,19 @@ def large_leg(steps):
             s.synchronize()
             dt = time.perf_counter() - t0
             traces[prec] = s.receivers().astype(np.float64)
+            pairs = s.time_block()[1]
         finally:
             s.close()
         res[prec] = {"gcell_per_s": round(w.cells * (steps - 1) / dt / 1e9, 1), "wall_s": round(dt, 2)}
-    ref = traces["fp64"]
-    for prec in ("fp32", "fp16", "bf16"):
-        res[prec]["rel_l2_vs_fp64"] = float(np.linalg.norm(traces[prec] - ref) / np.linalg.norm(ref))
+        if two:
+            res[prec]["pair_tiles"] = int(pairs)
+            if "fp32" in traces:
+                res[prec]["bitexact_vs_fp32"] = bool(np.array_equal(traces[prec], traces["fp32"]))
+    if "fp64" in traces:
+        ref = traces["fp64"]
+        for prec in ("fp32", "fp16", "bf16"):
+            if prec in traces:
+                res[prec]["rel_l2_vs_fp64"] = float(np.linalg.norm(traces[prec] - ref) / np.linalg.norm(ref))
     return res
 
 
@@ -245,6 +257,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-steps", type=int, default=20000)
+    ap.add_argument("--large-precisions", default="fp64,fp32,fp16,bf16,fp32_two_steps_per_pass",
+                    help="comma-separated runs of the large-room leg")
     ap.add_argument("--surrogate-steps", type=int, default=30000)
     ap.add_argument("--surrogate-eager", action="store_true", help="train without captured graphs")
     ap.add_argument("--no-zone", action="store_true")
@@ -314,7 +328,8 @@ def main():
                                f"4 IIR materials, 12 boxes, 12-cell PML x+, 16 receivers",
                    "mode": "compat" if args.compat else "physical",
                    "boundary_cells": int(st.boundary), "fused_step_kernel": bool(st.fused),
-                   "tile": [st.tile_x, st.tile_y, st.tile_z], "autotuned": tile is not None,
+                   "tile": [st.tile_x, st.tile_y, st.tile_z], "steps_per_pass": sim.time_block()[0],
+                   "autotuned": tile is not None,
                    "profile": {"step_ms": round(prof.step_ms, 5),
                                "kernel_ms": [round(x, 5) for x in prof.kernel_ms],
                                "launches": list(prof.launches)}},
@@ -328,7 +343,7 @@ def main():
     if not args.no_zone:
         out["surrogate_zone"] = zone_leg(args.zone_steps)
     if not args.no_large:
-        out["large"] = large_leg(args.large_steps)
+        out["large"] = large_leg(args.large_steps, tuple(args.large_precisions.split(",")))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
