@@ -165,3 +165,24 @@ def test_time_block_argument_checks():
     with pytest.raises(afx.AfxError):
         s.set_time_block(3)
     s.close()
+
+
+def test_profiled_steps_with_two_steps_per_pass():
+    """afx_profile_run (events around every launch, no graphs) with two steps
+    per pass: 33 steps (16 passes and one single step) continue the state
+    exactly like afx_run, and every step counts two launches per pass."""
+    _case("profile run with two steps per pass, 33 + 40 steps")
+    v, a, b, sc, mc, prof = _room()
+    steps = 73
+    src = _src(steps)
+    s = _sim(v, a, b, sc, mc, prof, src)
+    s.run(steps)
+    want = s.receivers()
+    s.close()
+    s = _sim(v, a, b, sc, mc, prof, src, tblock=2)
+    pr = s.profile(33)
+    assert pr.steps == 33 and pr.launches[0] == 33  # 16 passes of two launches, one single step
+    s.run(40)
+    got = s.receivers()
+    s.close()
+    _assert_bitexact(got, want, "traces")
