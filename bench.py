@@ -3,10 +3,15 @@
 A step is one leapfrog time step of the furnished room (256 x 192 x 128
 cells, 4 IIR wall materials, 12 seeded furniture boxes, 12-cell PML on one
 face, band-limited noise source, 16 receivers), FP32, physical boundary
-mode, run through the C ABI.  Prints one JSON line.
+mode, run through the C ABI.  Prints one JSON line; its extra legs are the
+48-room batch (batched), the learned surrogate over uniform positions
+(surrogate) and over a loudspeaker zone (surrogate_zone), and the
+640 x 480 x 320 room in every storage precision and with two steps per pass
+(large).
 
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
-                  [--compat] [--no-cpu-baseline] [--no-tune] [--no-batched] [--no-surrogate] [--no-large]
+                  [--compat] [--no-cpu-baseline] [--no-tune] [--no-batched] [--no-surrogate]
+                  [--no-zone] [--no-large] [--large-precisions LIST] [--copy-probe BYTES]
 """
 import argparse
 import json
