@@ -384,6 +384,7 @@ _MLP_SIGS = {
     "afx_mlp_set_params": (C.c_int, [_P, _f32p, C.c_int64]),
     "afx_mlp_get_frequencies": (C.c_int, [_P, _f32p, C.c_int32]),
     "afx_mlp_forward": (C.c_int, [_P, _f32p, C.c_int64, _f32p]),
+    "afx_mlp_forward_device": (C.c_int, [_P, C.c_void_p, C.c_int64, C.c_void_p]),
     "afx_mlp_gradients": (C.c_int, [_P, _f32p, _f32p, _f32p, C.c_int64, _f32p, C.POINTER(C.c_double), _f32p]),
     "afx_mlp_set_data": (C.c_int, [_P, _f32p, _f32p, _f32p, C.c_int64]),
     "afx_mlp_train": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, C.c_uint32, _f32p]),
@@ -453,6 +454,11 @@ class Mlp:
         _check(lib.afx_mlp_forward(self._h, xp, rows, y.ctypes.data_as(_f32p)))
         return y
 
+    def forward_device(self, x_dev, rows, y_dev):
+        """Outputs for `rows` queries already in device memory (DeviceBuffer
+        or raw device addresses); y_dev receives [rows][out_dim] FP32."""
+        _check(lib.afx_mlp_forward_device(self._h, int(x_dev), int(rows), int(y_dev)))
+
     def gradients(self, x, target, weight):
         x, xp = _f32(x)
         t, tp = _f32(target)
@@ -480,3 +486,64 @@ class Mlp:
         st = MlpStats()
         _check(lib.afx_mlp_get_stats(self._h, C.byref(st)))
         return st
+
+
+# ---- device memory for the device-pointer entry points ----------------------
+# The library's own HIP runtime (the one libafx.so links) allocates it, so no
+# second runtime enters the process.
+
+_hip = None
+
+
+def _hip_lib():
+    global _hip
+    if _hip is None:
+        h = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipSetDevice.argtypes = [C.c_int]
+        for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipSetDevice):
+            f.restype = C.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceBuffer:
+    """`nbytes` of device memory on `device`; int(buf) is its address."""
+
+    def __init__(self, nbytes, device=0):
+        h = _hip_lib()
+        if h.hipSetDevice(device) != 0:
+            raise AfxError(-1, f"hipSetDevice({device}) failed")
+        p = C.c_void_p()
+        if h.hipMalloc(C.byref(p), nbytes) != 0:
+            raise AfxError(-1, f"hipMalloc of {nbytes} bytes failed")
+        self.ptr, self.nbytes = p.value, nbytes
+
+    def __int__(self):
+        return self.ptr
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if _hip_lib().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) != 0:  # host to device
+            raise AfxError(-1, "hipMemcpy to device failed")
+
+    def download(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        if _hip_lib().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:  # device to host
+            raise AfxError(-1, "hipMemcpy to host failed")
+        return out
+
+    def close(self):
+        if self.ptr:
+            _hip_lib().hipFree(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -272,6 +272,80 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
     }
 }
 
+// Inference GEMM for large M (HIDDEN and OUT epilogues): a 64x64 output tile
+// per workgroup, operands staged through LDS in k-steps of 32 (double
+// buffered, one barrier per step), each wave a 32x32 quadrant.  It gives the
+// same bits as k_gemm: every MFMA sees the same operands per lane, and each
+// output keeps four partial sums, one per k-slice (k-step mod 4) as k_gemm's
+// waves do, added in k_gemm's order.  Needs M % 64 == 0, N % 64 == 0,
+// K % 128 == 0.
+template <int E>
+__global__ __launch_bounds__(256) void k_gemm64(Gemm g)
+{
+    constexpr int kPad = 40;  // LDS row stride in bf16 (32 + 8)
+    __shared__ bf16 As[2][64][kPad];
+    __shared__ bf16 Bs[2][64][kPad];
+    const int tiles_n = g.N >> 6;
+    const int m0 = (int)(blockIdx.x / tiles_n) << 6;
+    const int n0 = (int)(blockIdx.x % tiles_n) << 6;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int lr = tid >> 2, lc = (tid & 3) << 3;  // this thread's 16-byte piece of each k-step
+    const bf16* ga = g.A + (size_t)(m0 + lr) * g.lda + lc;
+    const bf16* gb = g.B + (size_t)(n0 + lr) * g.ldb + lc;
+    bf16x8 ra = *reinterpret_cast<const bf16x8*>(ga);
+    bf16x8 rb = *reinterpret_cast<const bf16x8*>(gb);
+    *reinterpret_cast<bf16x8*>(&As[0][lr][lc]) = ra;
+    *reinterpret_cast<bf16x8*>(&Bs[0][lr][lc]) = rb;
+    __syncthreads();
+    const int r = lane & 15, kq = (lane >> 4) << 3;
+    f32x4 p[4][2][2];
+    for (int sl = 0; sl < 4; ++sl)
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) p[sl][i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int nk = g.K >> 5;
+    for (int kt0 = 0; kt0 < nk; kt0 += 4) {
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+            const int kt = kt0 + sl, cur = kt & 1;
+            const bool more = kt + 1 < nk;
+            if (more) {
+                ra = *reinterpret_cast<const bf16x8*>(ga + 32 * (kt + 1));
+                rb = *reinterpret_cast<const bf16x8*>(gb + 32 * (kt + 1));
+            }
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&As[cur][32 * wi + r][kq]);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&As[cur][32 * wi + 16 + r][kq]);
+            const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(&Bs[cur][32 * wj + r][kq]);
+            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(&Bs[cur][32 * wj + 16 + r][kq]);
+            p[sl][0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, p[sl][0][0], 0, 0, 0);
+            p[sl][0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, p[sl][0][1], 0, 0, 0);
+            p[sl][1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, p[sl][1][0], 0, 0, 0);
+            p[sl][1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, p[sl][1][1], 0, 0, 0);
+            if (more) {
+                *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr][lc]) = ra;
+                *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr][lc]) = rb;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + 32 * wj + 16 * j + r;
+            const int mb = m0 + 32 * wi + 16 * i + ((lane >> 4) << 2);
+            for (int e = 0; e < 4; ++e) {
+                const float v = ((p[0][i][j][e] + p[1][i][j][e]) + p[2][i][j][e]) + p[3][i][j][e];
+                const int m = mb + e;
+                if (E == EPI_HIDDEN) {
+                    const float z = v + g.bias[n];
+                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
+                    g.act[(size_t)m * g.ldact + n] = (bf16)silu(z);
+                } else {
+                    g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+                }
+            }
+        }
+}
+
 __global__ __launch_bounds__(256) void k_gemm_grouped(GradTable t, AdamArgs ad, StepRef sr)
 {
     __shared__ float lds[4][4][4][64];
@@ -476,6 +550,14 @@ struct afx_mlp {
     float* Y = nullptr;             // [B][out]
     float *xq = nullptr, *tq = nullptr, *wq = nullptr;  // staging for forward/gradients
     float* partial = nullptr;       // loss partials of one step
+    // inference: chunks of up to kInferRows rows, own buffers (allocated on
+    // first use) so a forward pass is a few large GEMMs instead of many
+    // batch-sized ones
+    void* imem = nullptr;
+    bf16* Xi = nullptr;
+    bf16* Ai[2] = {nullptr, nullptr};  // hidden activations, ping-pong
+    float* Yi = nullptr;
+    float* xqi = nullptr;
     StepRef sr{};                   // device step position while training (null otherwise)
     int nblk_loss = 0;
     // training set
@@ -491,6 +573,17 @@ namespace {
 int gemm_launch(afx_mlp* s, int epi, Gemm g)
 {
     if (g.M % 32 || g.N % 32 || g.K % 32) return err(AFX_E_ARG, "gemm %dx%dx%d not a multiple of 32", g.M, g.N, g.K);
+    // large-M inference GEMMs on 64x64 tiles (same bits; see k_gemm64)
+    if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 64 == 0 && g.N % 64 == 0 &&
+        g.K % 128 == 0) {
+        const dim3 grid64((unsigned)((g.M / 64) * (g.N / 64))), block64(256);
+        if (epi == EPI_HIDDEN)
+            k_gemm64<EPI_HIDDEN><<<grid64, block64, 0, s->stream>>>(g);
+        else
+            k_gemm64<EPI_OUT><<<grid64, block64, 0, s->stream>>>(g);
+        HIP_CHECK(hipGetLastError());
+        return AFX_OK;
+    }
     const dim3 grid((unsigned)((g.M / 32) * (g.N / 32))), block(256);
     switch (epi) {
     case EPI_HIDDEN: k_gemm<EPI_HIDDEN><<<grid, block, 0, s->stream>>>(g); break;
@@ -674,6 +767,103 @@ void init_params(const afx_mlp* s, std::vector<float>& p)
 
 }  // namespace
 
+namespace {
+
+constexpr int kInferRows = 8192;
+
+int ensure_infer(afx_mlp* s)
+{
+    if (s->imem) return AFX_OK;
+    int wmax = 32;
+    for (int l = 0; l + 1 < s->L; ++l) wmax = std::max(wmax, s->out[l]);
+    Arena probe;
+    auto layout = [&](Arena& a) {
+        s->Xi = a.take<bf16>((size_t)kInferRows * s->F);
+        s->Ai[0] = a.take<bf16>((size_t)kInferRows * wmax);
+        s->Ai[1] = a.take<bf16>((size_t)kInferRows * wmax);
+        s->Yi = a.take<float>((size_t)kInferRows * s->cfg.out_dim);
+        s->xqi = a.take<float>((size_t)kInferRows * s->cfg.in_dim);
+    };
+    layout(probe);
+    if (hipMalloc(&s->imem, probe.used) != hipSuccess)
+        return err(AFX_E_NOMEM, "device allocation of %zu bytes for inference failed", probe.used);
+    Arena real;
+    real.base = static_cast<char*>(s->imem);
+    real.cap = probe.used;
+    layout(real);
+    return AFX_OK;
+}
+
+// Network outputs for the m rows (a multiple of 32) staged in xqi: features,
+// hidden layers through the ping-pong buffers, output layer into y (row
+// stride out_dim).  Each output row depends on its input row only, with the
+// same K order as the batch-sized training kernels, so chunking does not
+// change a value.
+int forward_chunk(afx_mlp* s, int m, float* y)
+{
+    const int I = s->cfg.in_dim, nf = s->cfg.n_freq, L = s->L;
+    const int nthreads = m * I * nf;
+    k_features<<<(nthreads + 255) / 256, 256, 0, s->stream>>>(s->xqi, nullptr, m, I, nf, s->freq, s->Xi, nullptr, m,
+                                                              StepRef{});
+    HIP_CHECK(hipGetLastError());
+    const bf16* act = s->Xi;
+    for (int l = 0; l < L; ++l) {
+        Gemm g{};
+        g.A = act;
+        g.lda = s->in[l];
+        g.B = s->wb[l];
+        g.ldb = s->in[l];
+        g.M = m;
+        g.N = s->out[l];
+        g.K = s->in[l];
+        g.bias = s->w + s->off[l] + (int64_t)s->out[l] * s->in[l];
+        if (l + 1 < L) {
+            g.act = s->Ai[l & 1];
+            g.ldact = s->out[l];
+            if (int rc = gemm_launch(s, EPI_HIDDEN, g)) return rc;
+            act = s->Ai[l & 1];
+        } else {
+            g.f32 = y;
+            g.ldf = s->out[l];
+            if (int rc = gemm_launch(s, EPI_OUT, g)) return rc;
+        }
+    }
+    return AFX_OK;
+}
+
+// Both forward entry points: x and y are host (dev = false) or device memory.
+int forward_rows(afx_mlp* s, const float* x, int64_t rows, float* y, bool dev)
+{
+    if (int rc = set_device(s)) return rc;
+    if (int rc = ensure_infer(s)) return rc;
+    const int I = s->cfg.in_dim, O = s->cfg.out_dim;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_CHECK(hipEventRecord(s->ev0, s->stream));
+    for (int64_t r0 = 0; r0 < rows; r0 += kInferRows) {
+        const int64_t nr = std::min<int64_t>(kInferRows, rows - r0);
+        const int m = (int)((nr + 63) / 64 * 64);
+        if (nr < m) HIP_CHECK(hipMemsetAsync(s->xqi + nr * I, 0, (size_t)(m - nr) * I * sizeof(float), s->stream));
+        HIP_CHECK(hipMemcpyAsync(s->xqi, x + r0 * I, (size_t)nr * I * sizeof(float),
+                                 dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream));
+        // a full device chunk is written in place; otherwise through Yi
+        const bool direct = dev && nr == m;
+        if (int rc = forward_chunk(s, m, direct ? y + r0 * O : s->Yi)) return rc;
+        if (!direct)
+            HIP_CHECK(hipMemcpyAsync(y + r0 * O, s->Yi, (size_t)nr * O * sizeof(float),
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_CHECK(hipEventRecord(s->ev1, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->st.last_forward_gpu_ms = ms;
+    s->st.last_forward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->st.last_forward_rows = rows;
+    return AFX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int afx_mlp_config_default(afx_mlp_config* c)
@@ -813,6 +1003,7 @@ void afx_mlp_destroy(afx_mlp* s)
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->mem) (void)hipFree(s->mem);
+    if (s->imem) (void)hipFree(s->imem);
     if (s->xd) (void)hipFree(s->xd);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -862,37 +1053,13 @@ int afx_mlp_get_frequencies(afx_mlp* s, float* out, int32_t cap)
 int afx_mlp_forward(afx_mlp* s, const float* x, int64_t rows, float* y)
 {
     if (!s || !x || !y || rows < 1) return err(AFX_E_ARG, "forward needs x, y and rows >= 1");
-    if (int rc = set_device(s)) return rc;
-    const int B = s->B, I = s->cfg.in_dim, O = s->cfg.out_dim, L = s->L;
-    const auto t0 = std::chrono::steady_clock::now();
-    HIP_CHECK(hipEventRecord(s->ev0, s->stream));
-    for (int64_t r0 = 0; r0 < rows; r0 += B) {
-        const int64_t nr = std::min<int64_t>(B, rows - r0);
-        if (nr < B) HIP_CHECK(hipMemsetAsync(s->xq, 0, (size_t)B * I * sizeof(float), s->stream));
-        HIP_CHECK(hipMemcpyAsync(s->xq, x + r0 * I, (size_t)nr * I * sizeof(float), hipMemcpyHostToDevice, s->stream));
-        if (int rc = forward_hidden(s, s->xq, nullptr, false)) return rc;
-        Gemm g{};
-        g.A = L >= 2 ? s->A[L - 2] : s->X;
-        g.lda = s->in[L - 1];
-        g.B = s->wb[L - 1];
-        g.ldb = s->in[L - 1];
-        g.M = B;
-        g.N = O;
-        g.K = s->in[L - 1];
-        g.bias = s->w + s->off[L - 1] + (int64_t)O * s->in[L - 1];
-        g.f32 = s->Y;
-        g.ldf = O;
-        if (int rc = gemm_launch(s, EPI_OUT, g)) return rc;
-        HIP_CHECK(hipMemcpyAsync(y + r0 * O, s->Y, (size_t)nr * O * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-    }
-    HIP_CHECK(hipEventRecord(s->ev1, s->stream));
-    HIP_CHECK(hipStreamSynchronize(s->stream));
-    float ms = 0.0f;
-    HIP_CHECK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    s->st.last_forward_gpu_ms = ms;
-    s->st.last_forward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    s->st.last_forward_rows = rows;
-    return AFX_OK;
+    return forward_rows(s, x, rows, y, false);
+}
+
+int afx_mlp_forward_device(afx_mlp* s, const float* x, int64_t rows, float* y)
+{
+    if (!s || !x || !y || rows < 1) return err(AFX_E_ARG, "forward needs x, y and rows >= 1");
+    return forward_rows(s, x, rows, y, true);
 }
 
 int afx_mlp_gradients(afx_mlp* s, const float* x, const float* target, const float* weight, int64_t rows,

@@ -106,6 +106,21 @@ def surrogate_leg(train_steps, eager=False):
         st = sur.net.stats()
         e_tr = sur.validate(ps, pr, tr_ir)
         fwd = sur.net.stats()
+        # inference with queries and responses resident in HBM: 65 536 rows
+        # (8 chunks of 8192) through afx_mlp_forward_device, best of three
+        from anc_field_py import _afx
+        rows_dev = 65536
+        xr = np.random.default_rng(0).uniform(0, 1, (rows_dev, 6)).astype(np.float32)
+        with _afx.DeviceBuffer(xr.nbytes) as xd, _afx.DeviceBuffer(rows_dev * sur.taps * 4) as yd:
+            xd.upload(xr)
+            inf_ms = 1e30
+            for _ in range(3):
+                sur.net.forward_device(xd, rows_dev, yd)
+                inf_ms = min(inf_ms, sur.net.stats().last_forward_gpu_ms)
+        cfg = sur.net.cfg
+        dims = [2 * cfg.in_dim * cfg.n_freq] + [cfg.width] * (cfg.depth - 1) + [cfg.out_dim]
+        flop_row = 2.0 * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+        inf_tflops = flop_row * rows_dev / (inf_ms * 1e-3) / 1e12
         e_ho = surrogate.rel_l2(sur.predict(hs, hr), ho_ir)
         e_nn = surrogate.rel_l2(surrogate.nearest_pair_baseline(ps, pr, tr_ir, hs, hr), ho_ir)
         runs = sur.fdtd_runs
@@ -128,6 +143,9 @@ def surrogate_leg(train_steps, eager=False):
             "heldout_rel_l2_median": float(np.median(e_ho)), "heldout_within_bound": float(np.mean(e_ho <= sur.bound)),
             "nearest_training_pair_heldout_rel_l2_median": float(np.median(e_nn)),
             "net_forward_impulse_responses_per_s": round(fwd.last_forward_rows / (fwd.last_forward_ms * 1e-3), 1),
+            "inference_device": {"rows": rows_dev, "gpu_ms": round(inf_ms, 3),
+                                 "impulse_responses_per_s": round(rows_dev / (inf_ms * 1e-3), 1),
+                                 "tflops": round(inf_tflops, 1), "frac_of_bf16_peak": round(inf_tflops / 2500.0, 4)},
             "query": {"pairs": int(len(ans)), "answered_by_net": int((~fb).sum()), "fdtd_fallbacks": int(fb.sum()),
                       "fdtd_source_runs": int(sur.fdtd_runs - runs), "wall_s": round(t_query, 3),
                       "max_rel_l2_vs_fdtd": float(np.max(e_ans))},

@@ -77,8 +77,15 @@ int afx_mlp_set_params(afx_mlp* mlp, const float* in, int64_t n);
 /* The n_freq frequencies of the feature ladder (cycles per unit coordinate). */
 int afx_mlp_get_frequencies(afx_mlp* mlp, float* out, int32_t cap);
 
-/* y[rows][out_dim] for x[rows][in_dim]; any rows >= 1 (processed in chunks of batch). */
+/* y[rows][out_dim] for x[rows][in_dim]; any rows >= 1.  Rows are processed
+ * in chunks of up to 8192 with inference buffers of their own (allocated on
+ * the first call); each output row depends on its input row only, so the
+ * values do not depend on the chunking or on the batch size. */
 int afx_mlp_forward(afx_mlp* mlp, const float* x, int64_t rows, float* y);
+/* The same with x and y in device memory of the handle's device, for callers
+ * whose queries and consumers live on the GPU (no host copies).  The call is
+ * ordered after work on the null stream and returns when y is written. */
+int afx_mlp_forward_device(afx_mlp* mlp, const float* x, int64_t rows, float* y);
 
 /* Gradient of the weighted loss over one batch (rows <= batch), without an
  * update: grad has the parameter layout; *loss receives the loss.  When y is

@@ -137,3 +137,42 @@ def test_mlp_graph_replay_matches_eager_training():
             out.append((m.params(), loss, loss2))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+def test_mlp_forward_chunking_and_device_path():
+    """Inference chunks of up to 8192 rows: 10 000 rows (a full chunk written
+    in place on the device path, then a padded 1808-row tail) give the same
+    bits as 300-row pieces, from host and from device memory."""
+    print("\n[case] mlp forward 10 000 rows: host, device, 300-row pieces", flush=True)
+    from anc_field_py import _afx
+    rng = np.random.default_rng(9)
+    n = 10000
+    x = rng.uniform(0, 1, (n, 6)).astype(np.float32)
+    with _afx.Mlp(seed=13) as m:
+        y = m.forward(x)
+        parts = np.concatenate([m.forward(x[i:i + 300]) for i in range(0, n, 300)])
+        assert np.array_equal(y, parts)
+        with _afx.DeviceBuffer(x.nbytes) as xd, _afx.DeviceBuffer(n * 1024 * 4) as yd:
+            xd.upload(x)
+            m.forward_device(xd, n, yd)
+            yg = yd.download((n, 1024))
+        assert np.array_equal(yg, y)
+        st = m.stats()
+        assert st.last_forward_rows == n and st.last_forward_gpu_ms > 0
+        print(f"device forward of {n} rows: {st.last_forward_gpu_ms:.3f} ms")
+
+
+def test_mlp_large_m_gemm_matches_training_kernels():
+    """Forward passes of 1024 rows or more run on 64x64 tiles staged through
+    LDS; their outputs equal, bit for bit, the outputs the training path's
+    split-K 32x32 kernels give for the same rows."""
+    print("\n[case] mlp 64x64 inference tiles vs split-K training kernels", flush=True)
+    from anc_field_py import _afx
+    rng = np.random.default_rng(10)
+    x = rng.uniform(0, 1, (2048, 6)).astype(np.float32)
+    t = (0.05 * rng.standard_normal((256, 1024))).astype(np.float32)
+    w = np.full(256, 1.0 / 256, np.float32)
+    with _afx.Mlp(seed=21) as m:
+        _, _, y_train = m.gradients(x[:256], t, w)
+        y_inf = m.forward(x)
+    assert np.array_equal(y_inf[:256], y_train)
