@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <unordered_map>
 #include <array>
@@ -2068,12 +2069,18 @@ bool can_pair(const afx_sim* s, int tile)
 struct TuneKey {
     std::vector<int32_t> dims;
     int prec, compat, pml;
+    uint32_t flags;  // launch path (split, list blocks, graphs)
+    int device;
     bool operator<(const TuneKey& o) const
     {
-        return std::tie(dims, prec, compat, pml) < std::tie(o.dims, o.prec, o.compat, o.pml);
+        return std::tie(dims, prec, compat, pml, flags, device) <
+               std::tie(o.dims, o.prec, o.compat, o.pml, o.flags, o.device);
     }
 };
+// Results shared by every handle of the process (handles may be driven from
+// different threads, so lookups and inserts hold the mutex; a trial does not).
 std::map<TuneKey, std::pair<int, int>> g_tune_cache;  // -> (tile, steps per pass)
+std::mutex g_tune_mutex;
 
 }  // namespace
 
@@ -2130,10 +2137,21 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
     key.prec = (int)s->prec;
     key.compat = s->compat;
     key.pml = s->pml_mode;
-    auto hit = g_tune_cache.find(key);
-    if (hit != g_tune_cache.end()) {
-        if (hit->second.first != s->tile || hit->second.second != s->tblock)
-            if (int rc = set_tile_tb(s, hit->second.first, hit->second.second)) return rc;
+    key.flags = s->cfg.flags;
+    key.device = s->cfg.device;
+    bool cached = false;
+    std::pair<int, int> choice{};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto hit = g_tune_cache.find(key);
+        if (hit != g_tune_cache.end()) {
+            cached = true;
+            choice = hit->second;
+        }
+    }
+    if (cached) {
+        if (choice.first != s->tile || choice.second != s->tblock)
+            if (int rc = set_tile_tb(s, choice.first, choice.second)) return rc;
     } else {
         Snapshot sn;
         if (int rc = snap_take(s, sn)) return rc;
@@ -2182,6 +2200,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         (void)hipEventDestroy(e1);
         if (int rc = snap_restore(s, sn)) return rc;
         if (int rc = set_tile_tb(s, best.first, best.second)) return rc;
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
         g_tune_cache[key] = best;
     }
     if (tile) {
