@@ -17,9 +17,12 @@
 //                      OUT     y = C + b (FP32)
 //                      LOSS    y = C + b; g = 2 w (y - t) -> BF16 both layouts; per-wave loss partial
 //                      BACK    g = C * silu'(z_prev) -> BF16 both layouts
-//   k_gemm_grouped   all weight gradients of a step in one launch: dW_l = dY_l^T . X_l
-//                    (A = dY_l^T [out][B], B = X_l^T [in][B], contraction over the batch)
-//   k_colsum         bias gradients, db_l[n] = sum_b dY_l[b][n], all layers in one launch
+//   k_gemm64<EPI>    HIDDEN and OUT for inference chunks: 64x64 tiles through LDS,
+//                    the same bits as k_gemm
+//   k_grads          all weight gradients of a step, dW_l = dY_l^T . X_l (A = dY_l^T
+//                    [out][B], B = X_l^T [in][B], contraction over the batch), and all
+//                    bias gradients, db_l[n] = sum_b dY_l[b][n], in one launch, each
+//                    with AdamW fused when training
 //   k_adam           AdamW on the FP32 master weights, 32x32 tiles of W per
 //                    workgroup; refreshes the BF16 copies W [out][in] (forward
 //                    operand) and W^T [in][out] (backward operand, written
@@ -68,13 +71,14 @@ enum Epi { EPI_HIDDEN = 0, EPI_OUT = 1, EPI_LOSS = 2, EPI_BACK = 3 };
 // it.  With step == null every offset is zero and the arguments are used as
 // given.
 struct StepRef {
-    const int64_t* step;
+    const int64_t* step;   // device counter, advanced once per run of steps
     int64_t rows_stride;   // minibatch rows advance per step
     int64_t part_stride;   // loss partials advance per step
     const float* sched;    // [step][3]: lr, 1 - beta1^t, 1 - beta2^t
+    int64_t rel;           // this launch's step after the counter's value
 };
 
-__device__ inline int64_t step_of(const StepRef& r) { return r.step ? *r.step : 0; }
+__device__ inline int64_t step_of(const StepRef& r) { return (r.step ? *r.step : 0) + r.rel; }
 
 struct Gemm {
     const bf16* A;       // [M][lda], K-contiguous
@@ -346,13 +350,15 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
         }
 }
 
-__global__ __launch_bounds__(256) void k_gemm_grouped(GradTable t, AdamArgs ad, StepRef sr)
+// Weight-gradient tile `bid` of the grouped problems (with AdamW fused when
+// ad.on).
+__device__ inline void grads_tile(const GradTable& t, AdamArgs ad, const StepRef& sr, int bid,
+                                  float (*lds)[4][4][64])
 {
-    __shared__ float lds[4][4][4][64];
     int p = 0;
-    while (p + 1 < t.n && (int)blockIdx.x >= t.start[p + 1]) ++p;
+    while (p + 1 < t.n && bid >= t.start[p + 1]) ++p;
     const GradProblem& g = t.p[p];
-    const int tile = (int)blockIdx.x - t.start[p];
+    const int tile = bid - t.start[p];
     const int tiles_n = g.N >> 5;
     const int m0 = (tile / tiles_n) << 5, n0 = (tile % tiles_n) << 5;
     f32x4 c[2][2];
@@ -402,10 +408,11 @@ __global__ void k_features(const float* __restrict__ x, const int* __restrict__ 
     }
 }
 
-// Bias gradients: one wave per output row of the flattened layer range.
-__global__ __launch_bounds__(256) void k_colsum(ColsumTable t, AdamArgs ad, StepRef sr)
+// Bias gradients of workgroup `bid`: one wave per output row of the
+// flattened layer range (with AdamW fused when ad.on).
+__device__ inline void colsum_rows(const ColsumTable& t, AdamArgs ad, const StepRef& sr, int bid)
 {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int row = bid * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= t.start[t.n]) return;
     int l = 0;
@@ -422,6 +429,19 @@ __global__ __launch_bounds__(256) void k_colsum(ColsumTable t, AdamArgs ad, Step
     }
     adam_sched(ad, sr);
     adam_step(ad.w, ad.m1, ad.m2, t.boff[l] + n, s, ad.lr, ad.b1, ad.b2, ad.eps, ad.wd, ad.bc1, ad.bc2);
+}
+
+// All weight gradients (workgroups [0, G), G = gt.start[gt.n]) and all bias
+// gradients (the rest) of one step in one launch; the two parts are
+// independent.
+__global__ __launch_bounds__(256) void k_grads(GradTable gt, ColsumTable ct, AdamArgs ad, StepRef sr)
+{
+    __shared__ float lds[4][4][4][64];
+    const int bid = (int)blockIdx.x, ng = gt.start[gt.n];
+    if (bid < ng)
+        grads_tile(gt, ad, sr, bid, lds);
+    else
+        colsum_rows(ct, ad, sr, bid - ng);
 }
 
 // AdamW on parameter i; returns its new value (the old one when !update).
@@ -489,9 +509,9 @@ __global__ __launch_bounds__(256) void k_adam(AdamTable t, float* __restrict__ w
     *reinterpret_cast<bf16x4*>(t.wt[l] + (size_t)(c0 + rr) * rows + r0 + cc) = o;
 }
 
-__global__ void k_step_inc(int64_t* step)
+__global__ void k_step_inc(int64_t* step, int64_t k)
 {
-    if (threadIdx.x == 0) *step += 1;
+    if (threadIdx.x == 0) *step += k;
 }
 
 #define HIP_CHECK(expr)                                                                                  \
@@ -698,8 +718,6 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
         ad = AdamArgs{s->w, s->m1, s->m2, 0.0f, s->cfg.beta1, s->cfg.beta2, s->cfg.eps, s->cfg.weight_decay,
                       1.0f, 1.0f, 1};
     }
-    k_gemm_grouped<<<gt.start[L], 256, 0, s->stream>>>(gt, ad, s->sr);
-    HIP_CHECK(hipGetLastError());
     ColsumTable ct{};
     ct.n = L;
     ct.ld = s->B;
@@ -712,7 +730,7 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
         ct.rows[l] = s->out[l];
         ct.start[l + 1] = ct.start[l] + s->out[l];
     }
-    k_colsum<<<(ct.start[L] + 3) / 4, 256, 0, s->stream>>>(ct, ad, s->sr);
+    k_grads<<<gt.start[L] + (ct.start[L] + 3) / 4, 256, 0, s->stream>>>(gt, ct, ad, s->sr);
     HIP_CHECK(hipGetLastError());
     return AFX_OK;
 }
@@ -1179,12 +1197,11 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
         return cleanup(err(AFX_E_HIP, "training schedule upload failed"));
     s->partial = d_part;
     s->sr = StepRef{d_step, (int64_t)B, (int64_t)s->nblk_loss, d_sched};
-    // One step: forward, loss, backward, AdamW, then the device step counter.
-    auto enqueue_step = [&]() -> int {
-        if (int rc = step_grad(s, s->xd, s->td, s->wd, d_idx, nullptr, true)) return rc;
-        k_step_inc<<<1, 64, 0, s->stream>>>(d_step);
-        HIP_CHECK(hipGetLastError());
-        return AFX_OK;
+    // One step (forward, loss, backward, AdamW) at step counter + rel; a
+    // captured chunk advances the counter once, after its last step.
+    auto enqueue_step = [&](int64_t rel) -> int {
+        s->sr.rel = rel;
+        return step_grad(s, s->xd, s->td, s->wd, d_idx, nullptr, true);
     };
     // Steps are replayed from a captured graph of kChunk steps; the rest run eagerly.
     constexpr int64_t kChunk = 32;
@@ -1194,7 +1211,11 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
         if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return cleanup(err(AFX_E_HIP, "graph capture failed to start"));
         int rc = AFX_OK;
-        for (int64_t k = 0; k < kChunk && !rc; ++k) rc = enqueue_step();
+        for (int64_t k = 0; k < kChunk && !rc; ++k) rc = enqueue_step(k);
+        if (!rc) {
+            k_step_inc<<<1, 64, 0, s->stream>>>(d_step, kChunk);
+            if (hipGetLastError() != hipSuccess) rc = err(AFX_E_HIP, "step counter launch failed");
+        }
         const hipError_t ce = hipStreamEndCapture(s->stream, &g);
         if (rc || ce != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
@@ -1208,8 +1229,9 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
     if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
     for (int64_t c = 0; c < nfull; ++c)
         if (hipGraphLaunch(exec, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "graph launch failed"));
+    // the rest eagerly, at the counter the chunks left (nfull * kChunk)
     for (int64_t k = nfull * kChunk; k < steps; ++k)
-        if (int rc = enqueue_step()) return cleanup(rc);
+        if (int rc = enqueue_step(k - nfull * kChunk)) return cleanup(rc);
     if (hipEventRecord(s->ev1, s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "event record failed"));
     if (hipStreamSynchronize(s->stream) != hipSuccess) return cleanup(err(AFX_E_HIP, "training failed on the device"));
     float ms = 0.0f;
