@@ -154,7 +154,7 @@ def surrogate_leg(train_steps, eager=False):
         sur.close()
 
 
-def zone_leg(train_steps, n_val=32, n_test=32):
+def zone_leg(train_steps, n_val=32, n_test=32, members=2):
     """Learned surrogate over a loudspeaker zone of the furnished room: a
     9^3-cell box of candidate source cells, 16 fixed receivers.  FDTD from
     the box's lattice of every 2nd cell (125 sources) trains an ensemble of
@@ -170,7 +170,7 @@ def zone_leg(train_steps, n_val=32, n_test=32):
     lo = surrogate.zone_box(w, rng, size)
     mics = surrogate.free_cells(w, rng, 16)
     zone = surrogate.SourceZone(lo, size, step, mics)
-    sur = surrogate.Surrogate(w, seed=1, members=2, freq_min=0.25, freq_max=16.0)
+    sur = surrogate.Surrogate(w, seed=1, members=members, freq_min=0.25, freq_max=16.0)
     try:
         t = time.perf_counter()
         loss = surrogate.fit_zone(sur, zone, train_steps, 1e-3, 1e-5, seed=3)
