@@ -212,13 +212,14 @@ def zone_leg(train_steps, n_val=32, n_test=32, members=2):
         sur.close()
 
 
-def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps_per_pass")):
+def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps_per_pass",
+                                 "fp64_two_steps_per_pass")):
     """Large room, reduced-precision storage: 640 x 480 x 320 cells (98 M),
     dx 0.01 m, 6 IIR materials, 16-cell PML on x+ and y+, band-limited
     noise, 32 receivers.  The same run in FP64, FP32, FP16 and BF16 storage;
-    receiver-trace error of each against FP64.  FP32 once more with two steps
-    per pass (time-step blocking), whose traces must equal FP32's bit for
-    bit."""
+    receiver-trace error of each against FP64.  FP32 and FP64 once more with
+    two steps per pass (time-step blocking), whose traces must equal the
+    single-step ones bit for bit."""
     import numpy as np
     from anc_field_py import _afx, rooms
     w = rooms.large(steps=steps)
@@ -226,7 +227,8 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
     traces = {}
     for prec in precisions:
         two = prec.endswith("two_steps_per_pass")
-        s = rooms.make_sim(w, _afx.Sim, precision="fp32" if two else prec)
+        base = prec.split("_")[0]
+        s = rooms.make_sim(w, _afx.Sim, precision=base)
         try:
             if two:
                 s.set_time_block(2)
@@ -243,8 +245,8 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
         res[prec] = {"gcell_per_s": round(w.cells * (steps - 1) / dt / 1e9, 1), "wall_s": round(dt, 2)}
         if two:
             res[prec]["pair_tiles"] = int(pairs)
-            if "fp32" in traces:
-                res[prec]["bitexact_vs_fp32"] = bool(np.array_equal(traces[prec], traces["fp32"]))
+            if base in traces:
+                res[prec][f"bitexact_vs_{base}"] = bool(np.array_equal(traces[prec], traces[base]))
     if "fp64" in traces:
         ref = traces["fp64"]
         for prec in ("fp32", "fp16", "bf16"):
@@ -280,7 +282,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-steps", type=int, default=20000)
-    ap.add_argument("--large-precisions", default="fp64,fp32,fp16,bf16,fp32_two_steps_per_pass",
+    ap.add_argument("--large-precisions", default="fp64,fp32,fp16,bf16,fp32_two_steps_per_pass,fp64_two_steps_per_pass",
                     help="comma-separated runs of the large-room leg")
     ap.add_argument("--surrogate-steps", type=int, default=30000)
     ap.add_argument("--surrogate-eager", action="store_true", help="train without captured graphs")
