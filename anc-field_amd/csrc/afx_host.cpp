@@ -208,7 +208,7 @@ struct afx_sim {
     int pml_rot = 0;
     // graph
     std::vector<std::pair<uint32_t, hipGraphExec_t>> graphs;  // by buffer rotation and pass mode
-    int graph_steps = 30;
+    int graph_steps = 120;  // a multiple of 3: buffer roles return after each graph
     int64_t group_count[26] = {0};
     double last_run_ms = 0.0;
     double last_run_gpu_ms = 0.0;
@@ -1406,9 +1406,9 @@ int run_steps(afx_sim* s, int64_t steps)
     if (rc) return rc;
     const bool graphs = !(s->cfg.flags & AFX_NO_GRAPH) && s->pml_mode != 1;
     const bool pairs = pairs_active(s);
-    // steps per graph: buffer roles return to the start after 30 single
-    // steps (a multiple of 3) or 16 two-step passes
-    const int K = pairs ? 32 : s->graph_steps;
+    // steps per graph: buffer roles return to the start after 120 single
+    // steps (a multiple of 3) or 64 two-step passes (a multiple of 4)
+    const int K = pairs ? 128 : s->graph_steps;
     int64_t done = 0;
     while (done < steps) {
         const int64_t left = steps - done;
@@ -1444,7 +1444,9 @@ int run_steps(afx_sim* s, int64_t steps)
             s->t += K;
             done += K;
         } else {
-            const int k = (int)std::min<int64_t>(left, graphs ? (pairs ? (int64_t)2 : (int64_t)1) : (int64_t)64);
+            // the rest (and runs without graphs) as eager launches, one step
+            // counter update per call
+            const int k = (int)std::min<int64_t>(left, 64);
             rc = enqueue_steps(s, k, done);
             if (rc) return rc;
             done += k;

@@ -96,9 +96,9 @@ def test_two_steps_per_pass_match_oracle(tile):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp16", "bf16"])
 def test_two_steps_per_pass_every_precision(precision):
-    _case(f"two vs one step per pass, {precision}, 75 steps, no graphs and graphs")
+    _case(f"two vs one step per pass, {precision}, 141 steps, no graphs and graphs")
     v, a, b, sc, mc, prof = _room()
-    steps = 75
+    steps = 141  # one 128-step graph of two-step passes, then 6 passes and a single step
     src = _src(steps)
     out = []
     for tblock, graph in ((1, True), (2, True), (2, False)):
