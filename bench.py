@@ -47,6 +47,51 @@ def cpu_baseline(w, budget_s=12.0):
                       f"FP32 oracle (C, OpenMP) on {cores} host threads"}
 
 
+def configs_leg():
+    """The single-room configurations over their full runs: the 64x48x32
+    shoebox (8-cell PML on x+, Ricker source, 4 receivers, 2000 steps) and
+    the furnished room (10 000 steps).  FP32 throughput, FP32 receiver error
+    against FP64, and for the shoebox the CPU oracle over the same 2000 steps
+    on this host's cores, its traces compared with the GPU's bit for bit."""
+    import numpy as np
+    import orc
+    from anc_field_py import _afx, rooms
+    out = {}
+    for name, w in (("shoebox_64x48x32_2000_steps", rooms.shoebox_small(steps=2000)),
+                    ("furnished_256x192x128_10000_steps", rooms.furnished(steps=10000))):
+        tr, gps = {}, 0.0
+        for prec in ("fp32", "fp64"):
+            s = rooms.make_sim(w, _afx.Sim, precision=prec)
+            try:
+                s.run(1)
+                s.synchronize()
+                t0 = time.perf_counter()
+                s.run(w.steps - 1)
+                s.synchronize()
+                dt = time.perf_counter() - t0
+                tr[prec] = s.receivers()
+            finally:
+                s.close()
+            if prec == "fp32":
+                gps = w.cells * (w.steps - 1) / dt / 1e9
+        a, b = tr["fp32"].astype(np.float64), tr["fp64"].astype(np.float64)
+        ent = {"cells": w.cells, "steps": w.steps, "fp32_gcell_per_s": round(gps, 1),
+               "fp32_hbm_roofline_frac_algorithmic": round(gps * 12 / HBM_PEAK_GBS, 4),
+               "rel_l2_fp32_vs_fp64": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+        if w.cells < 10 ** 6:
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            o = orc.from_workload(w, compat=False, threads=cores)
+            t0 = time.perf_counter()
+            o.run(w.steps)
+            dt = time.perf_counter() - t0
+            ent["cpu_oracle_gcell_per_s"] = round(w.cells * w.steps / dt / 1e9, 3)
+            ent["cpu_cores"] = cores
+            ent["gpu_bitexact_vs_cpu_oracle"] = bool(np.array_equal(np.asarray(o.receivers(), np.float32), tr["fp32"]))
+            o.close()
+        out[name] = ent
+    return out
+
+
 def batched_leg(steps):
     """Batched identification: 48 seeded rooms (64^3 .. 256x192x128, 2-4 IIR
     materials, one Ricker source and 8 receivers each) in one handle, against
@@ -287,6 +332,7 @@ def main():
     ap.add_argument("--surrogate-steps", type=int, default=30000)
     ap.add_argument("--surrogate-eager", action="store_true", help="train without captured graphs")
     ap.add_argument("--no-zone", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the full single-room runs")
     ap.add_argument("--zone-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -361,6 +407,8 @@ def main():
         "roofline": roof,
     }
     sim.close()
+    if not args.no_configs:
+        out["configs"] = configs_leg()
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_steps)
     if not args.no_surrogate:
