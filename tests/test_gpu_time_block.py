@@ -186,3 +186,46 @@ def test_profiled_steps_with_two_steps_per_pass():
     got = s.receivers()
     s.close()
     _assert_bitexact(got, want, "traces")
+
+
+@pytest.mark.parametrize("ncoef", [1, 3, 4])
+def test_two_steps_per_pass_compat_mode_match_oracle(ncoef):
+    """The reference's compat mode (lambda^2 at the boundary, the no-update
+    corner group) with IIR filters of 1, 3 and 4 coefficients, no PML: two
+    steps per pass against the oracle, bit for bit, over 131 steps (one
+    128-step graph, a pass and a single step)."""
+    _case(f"two steps per pass, compat mode, ncoef={ncoef}, 131 steps")
+    rng = np.random.Generator(np.random.PCG64(50 + ncoef))
+    nx, ny, nz = 264, 40, 70
+    if ncoef == 3:
+        a, b = rooms.materials(rng, 3)
+    else:
+        a = (rng.uniform(0.02, 0.3, (3, ncoef)) * np.array([1.0] + [0.3] * (ncoef - 1))).astype(np.float32)
+        b = np.zeros((3, ncoef), np.float32)
+        b[:, 0] = 1.0
+        if ncoef > 1:
+            b[:, 1:] = rng.uniform(-0.2, 0.2, (3, ncoef - 1))
+    v = rooms.shoebox(nx, ny, nz, [int(rng.integers(0, 3)) for _ in range(6)])
+    v = rooms.repair(rooms.add_boxes(v, rng, 3, 4, 9, 3))
+    inner = rooms.interior_cells(v, rng, 3)
+    sc = np.stack([inner[0], np.array([0, 5, 5], np.int32)])
+    mc = np.stack([inner[1], inner[2], np.array([0, 5, 5], np.int32)])
+    steps = 131
+    dt, c = 3.125e-05, 340.0
+    src = np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 800.0, amp=0.5)])
+    s = afx.Sim(c=c, dx=0.02, dt=dt, res=0.02, compat=True)
+    s.set_voxels(v)
+    s.set_materials(a, b)
+    s.set_time_block(2)
+    for cell, sig in zip(sc, src):
+        s.add_source_cell(cell, sig)
+    for cell in mc:
+        s.add_receiver_cell(cell)
+    s.run(steps)
+    got, f, npair = s.receivers(), s.pressure(0), s.time_block()[1]
+    s.close()
+    assert npair > 0
+    o = orc.Oracle(v, a, b, c, 0.02, dt, True, sc, src, mc, "f32", 0, 0, 0, None)
+    o.run(steps)
+    _assert_bitexact(got, o.receivers(), "traces")
+    _assert_bitexact(f, o.field(0), "field")
