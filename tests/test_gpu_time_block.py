@@ -229,3 +229,21 @@ def test_two_steps_per_pass_compat_mode_match_oracle(ncoef):
     o.run(steps)
     _assert_bitexact(got, o.receivers(), "traces")
     _assert_bitexact(f, o.field(0), "field")
+
+
+@pytest.mark.parametrize("kw", [dict(fused=False), dict(list_blocks=True)])
+def test_two_steps_per_pass_falls_back_where_it_cannot_apply(kw):
+    """The split path and the list-blocks launch take single steps even with
+    two steps per pass requested (no tile pairs), with unchanged results."""
+    _case(f"two steps per pass requested with {kw}: single steps, same traces")
+    v, a, b, sc, mc, prof = _room()
+    steps = 40
+    src = _src(steps)
+    out = []
+    for tblock in (1, 2):
+        s = _sim(v, a, b, sc, mc, prof, src, tblock=tblock, **kw)
+        s.run(steps)
+        out.append((s.receivers(), s.time_block()))
+        s.close()
+    assert out[1][1] == (2, 0)
+    _assert_bitexact(out[1][0], out[0][0], "traces")
