@@ -24,6 +24,12 @@ sys.path.insert(0, os.path.join(ROOT, "anc-field_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def progress(msg):
+    """One line on stderr as each part of the run ends (stdout carries only
+    the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 BYTES_PER_CELL = {"fp32": 12, "fp64": 24, "fp16": 6, "bf16": 6}
 
 
@@ -74,6 +80,7 @@ def configs_leg():
                 s.close()
             if prec == "fp32":
                 gps = w.cells * (w.steps - 1) / dt / 1e9
+            progress(f"configs: {name} {prec} done")
         a, b = tr["fp32"].astype(np.float64), tr["fp64"].astype(np.float64)
         ent = {"cells": w.cells, "steps": w.steps, "fp32_gcell_per_s": round(gps, 1),
                "fp32_hbm_roofline_frac_algorithmic": round(gps * 12 / HBM_PEAK_GBS, 4),
@@ -112,8 +119,10 @@ def batched_leg(steps):
                                            "kind": "separate launches per step, no graphs",
                                            "bitexact_vs_fused": bool((seq_split == seq).all())}}
     runs = {}
+    progress("batched: sequential runs done")
     for prec in ("fp32", "fp16", "bf16", "fp64"):
         runs[prec] = batch.run_batched(ws, steps, prec, tune=True)
+        progress(f"batched: {prec} done")
     ref = runs["fp64"][0]
     for prec, (tr, t, tile) in runs.items():
         res[f"batched_{prec}"] = {
@@ -143,6 +152,7 @@ def surrogate_leg(train_steps, eager=False):
     try:
         t = time.perf_counter()
         ir = sur.simulate(src, rcv)
+        progress("surrogate: FDTD responses done")
         t_fdtd = time.perf_counter() - t
         ps, pr = surrogate.pairs(src[:64], rcv[:64])
         hs, hr = surrogate.pairs(src[64:], rcv[64:])
@@ -219,12 +229,14 @@ def zone_leg(train_steps, n_val=32, n_test=32, members=2):
     try:
         t = time.perf_counter()
         loss = surrogate.fit_zone(sur, zone, train_steps, 1e-3, 1e-5, seed=3)
+        progress("surrogate zone: lattice FDTD and training done")
         t_fit = time.perf_counter() - t
         cells = lo + np.stack(np.meshgrid(*[np.arange(size)] * 3, indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
         off = cells[np.any((cells - lo) % step, axis=1)]
         pick = off[rng.permutation(len(off))]
         val, test = pick[:n_val], pick[n_val:n_val + n_test]
         e_val, t_val, sp_val = surrogate.calibrate_zone(sur, zone, val)
+        progress("surrogate zone: calibration done")
         ts, tr_ = surrogate.pairs(test, [mics] * len(test))
         truth = sur.simulate(test, [mics] * len(test))
         net = sur.predict(ts, tr_)
@@ -288,6 +300,7 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
         finally:
             s.close()
         res[prec] = {"gcell_per_s": round(w.cells * (steps - 1) / dt / 1e9, 1), "wall_s": round(dt, 2)}
+        progress(f"large: {prec} done")
         if two:
             res[prec]["pair_tiles"] = int(pairs)
             if base in traces:
@@ -407,14 +420,17 @@ def main():
         "roofline": roof,
     }
     sim.close()
+    progress("main leg done")
     if not args.no_configs:
         out["configs"] = configs_leg()
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_steps)
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(args.surrogate_steps, args.surrogate_eager)
+        progress("surrogate done")
     if not args.no_zone:
         out["surrogate_zone"] = zone_leg(args.zone_steps)
+        progress("surrogate zone done")
     if not args.no_large:
         out["large"] = large_leg(args.large_steps, tuple(args.large_precisions.split(",")))
     if not args.no_cpu_baseline:
