@@ -66,9 +66,11 @@ typedef struct {
     /* absorbing layer: 0 none, 1 compat pml.cl, 2 physical (Grote-Sim) */
     int32_t pml_mode;
     int32_t pml_width;
-    const float* ksi[3];      /* per-axis profiles: compat: kappa[nx|ny|nz];
-                                 physical: zeta at cell centres */
-    const float* ksi_h[3];    /* physical: zeta at +1/2 points */
+    const double* ksi[3];     /* per-axis profiles: compat: kappa[nx|ny|nz];
+                                 physical: zeta at cell centres.  Double, so
+                                 the f64 instantiation keeps the derived zeta
+                                 unrounded; f32 rounds it once on load */
+    const double* ksi_h[3];   /* physical: zeta at +1/2 points */
     const int32_t* pml_boxes; /* physical: [nbox][6] x0,x1,y0,y1,z0,z1 (half-open), disjoint */
     int32_t pml_nbox;
 } orc_problem;

@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
 
 _f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
 
 
@@ -26,7 +27,7 @@ class Problem(C.Structure):
         ("nsrc", C.c_int32), ("src_cell", _i32p), ("src", _f32p), ("src_len", C.c_int64),
         ("nmic", C.c_int32), ("mic_cell", _i32p),
         ("pml_mode", C.c_int32), ("pml_width", C.c_int32),
-        ("ksi", _f32p * 3), ("ksi_h", _f32p * 3),
+        ("ksi", _f64p * 3), ("ksi_h", _f64p * 3),
         ("pml_boxes", _i32p), ("pml_nbox", C.c_int32),
     ]
 
@@ -74,7 +75,7 @@ def compat_ksi(dims, profile):
     dmax = max(dims) + 2
     out = []
     for n in dims:
-        k = np.zeros(dmax, np.float32)
+        k = np.zeros(dmax, np.float64)
         L = len(profile)
         for i in range(min(L, n)):
             k[i] = profile[L - 1 - i]
@@ -119,8 +120,8 @@ def phys_pml(dims, width, faces, profile):
     for d in range(3):
         n = dims[d]
         lo_on, hi_on = (faces >> (2 * d)) & 1, (faces >> (2 * d + 1)) & 1
-        c_arr = np.zeros(n + 1, np.float32)
-        h_arr = np.zeros(n + 1, np.float32)
+        c_arr = np.zeros(n + 1, np.float64)
+        h_arr = np.zeros(n + 1, np.float64)
         for i in range(n):
             vc = vh = 0.0
             if hi_on:
@@ -199,13 +200,13 @@ class Oracle:
         if pml_mode == 1:
             ks = compat_ksi((nx, ny, nz), [] if pml_profile is None else list(np.asarray(pml_profile, np.float32)))
             for d in range(3):
-                pb.ksi[d] = ks[d].ctypes.data_as(_f32p)
+                pb.ksi[d] = ks[d].ctypes.data_as(_f64p)
             self._keep += ks
         elif pml_mode == 2:
             zc, zh, boxes = phys_pml((nx, ny, nz), pml_width, pml_faces, pml_profile)
             for d in range(3):
-                pb.ksi[d] = zc[d].ctypes.data_as(_f32p)
-                pb.ksi_h[d] = zh[d].ctypes.data_as(_f32p)
+                pb.ksi[d] = zc[d].ctypes.data_as(_f64p)
+                pb.ksi_h[d] = zh[d].ctypes.data_as(_f64p)
             boxes = np.ascontiguousarray(boxes)
             pb.pml_boxes = boxes.ctypes.data_as(_i32p)
             pb.pml_nbox = boxes.shape[0]

@@ -135,9 +135,12 @@ def test_resumable_runs():
     _assert_bitexact(f, wf, "field")
 
 
-@pytest.mark.parametrize("fused,list_blocks", [(True, False), (True, True), (False, False)])
-def test_physical_pml_bitexact(fused, list_blocks):
-    _case(f"physical PML x+ and y- faces fused={fused} list_blocks={list_blocks}")
+@pytest.mark.parametrize("fused,list_blocks,precision", [(True, False, "fp32"), (True, True, "fp32"),
+                                                          (False, False, "fp32"), (True, False, "fp64"),
+                                                          (False, False, "fp64")])
+def test_physical_pml_bitexact(fused, list_blocks, precision):
+    """FP64 keeps the zeta tables in double, as the oracle's f64 build does."""
+    _case(f"physical PML x+ and y- faces fused={fused} list_blocks={list_blocks} {precision}")
     v, a, b, rng = _closed_room(23, dims=(48, 40, 30))
     v[:, :, -1] = -1   # open x+ face
     v[:, 0, :] = -1    # open y- face
@@ -153,7 +156,7 @@ def test_physical_pml_bitexact(fused, list_blocks):
     src = np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 900.0, amp=0.5),
                     rooms.ricker(steps, dt, 1300.0, amp=0.3)])
     prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
-    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, pml=(6, 2 | 4, prof),
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, precision=precision, pml=(6, 2 | 4, prof),
                                  lib_kw=dict(fused=fused, list_blocks=list_blocks))
     _assert_bitexact(got, want, "traces")
     _assert_bitexact(f, wf, "field")
