@@ -189,7 +189,7 @@ struct afx_sim {
     int64_t npair = 0;      // tiles advanced two steps per pass
     int32_t npml = 0;
     int32_t ntiles = 0;
-    DBuf d_e_cell, d_e_info, d_e_mat, d_ring_in, d_ring_out, d_A, d_B;
+    DBuf d_e_cell, d_e_info, d_e_mat, d_ring_in, d_ring_out, d_A, d_B, d_M;
     int32_t nb = 0;
     std::vector<Entry> entries;
     DBuf d_sp_cell, d_sp_room, d_sp_bidx, d_sp_src_off, d_sp_src, d_sp_mic_off, d_sp_mic;
@@ -932,6 +932,31 @@ int setup_pml_t(afx_sim* s)
     return AFX_OK;
 }
 
+// lambda^2 in storage-arithmetic precision exactly as simulation.hpp:166
+static void step_scalars(const afx_sim* s, StepArgs& a)
+{
+    if (s->prec == Prec::F64) {
+        const double c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
+        const double lam = dt * c / dx;
+        a.l2 = lam * lam;
+        a.c2 = 2.0 * (1.0 - 3.0 * a.l2);
+        a.lam_b = s->compat ? a.l2 : lam;
+        a.l2b = a.lam_b * a.lam_b;
+        a.c2b = 2.0 * (1.0 - 3.0 * a.l2b);
+    } else {
+        const float c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
+        const float lam = dt * c / dx;
+        const float l2 = (dt * c / dx) * (dt * c / dx);
+        const float lb = s->compat ? l2 : lam;
+        const float l2b = lb * lb;
+        a.l2 = l2;
+        a.c2 = 2.0f * (1.0f - 3.0f * l2);
+        a.lam_b = lb;
+        a.l2b = l2b;
+        a.c2b = 2.0f * (1.0f - 3.0f * l2b);
+    }
+}
+
 int commit(afx_sim* s)
 {
     if (s->committed) {
@@ -1052,16 +1077,27 @@ int commit(afx_sim* s)
     HIP_TRY(s->d_ring_out.alloc(rbytes));
     HIP_TRY(launch_fill(s->d_ring_in.p, (int64_t)rbytes, s->stream));
     HIP_TRY(launch_fill(s->d_ring_out.p, (int64_t)rbytes, s->stream));
+    int nmat = 1;
     if (s->prec == Prec::F64) {
         std::vector<double> a(s->A.begin(), s->A.end()), b(s->B.begin(), s->B.end());
         if (a.empty()) { a.push_back(1.0); b.push_back(1.0); }
+        nmat = (int)(a.size() / (size_t)N);
         HIP_TRY(upload(s->d_A, a, s->stream));
         HIP_TRY(upload(s->d_B, b, s->stream));
     } else {
         std::vector<float> a = s->A, b = s->B;
         if (a.empty()) { a.push_back(1.0f); b.push_back(1.0f); }
+        nmat = (int)(a.size() / (size_t)N);
         HIP_TRY(upload(s->d_A, a, s->stream));
         HIP_TRY(upload(s->d_B, b, s->stream));
+    }
+    {
+        // per-material quotients of the boundary update, from the uploaded
+        // coefficients in the compute type
+        StepArgs sc{};
+        step_scalars(s, sc);
+        HIP_TRY(s->d_M.alloc(cbytes * 4 * (size_t)std::max(nmat, 1)));
+        HIP_TRY(launch_mat_consts(s->prec, s->d_A.p, s->d_B.p, N, nmat, sc.lam_b, sc.l2b, s->d_M.p, s->stream));
     }
     // fields
     const size_t fbytes = (prec_bytes(s->prec) * (size_t)s->total_cells + 3) / 4 * 4;
@@ -1121,27 +1157,7 @@ StepArgs make_args(afx_sim* s)
     a.ntiles = s->ntiles;
     a.rooms = s->d_rooms.as<RoomDev>();
     a.room0 = s->room0;
-    // lambda^2 in storage-arithmetic precision exactly as simulation.hpp:166
-    if (s->prec == Prec::F64) {
-        const double c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
-        const double lam = dt * c / dx;
-        a.l2 = lam * lam;
-        a.c2 = 2.0 * (1.0 - 3.0 * a.l2);
-        a.lam_b = s->compat ? a.l2 : lam;
-        a.l2b = a.lam_b * a.lam_b;
-        a.c2b = 2.0 * (1.0 - 3.0 * a.l2b);
-    } else {
-        const float c = s->cfg.c, dx = s->cfg.dx, dt = s->cfg.dt;
-        const float lam = dt * c / dx;
-        const float l2 = (dt * c / dx) * (dt * c / dx);
-        const float lb = s->compat ? l2 : lam;
-        const float l2b = lb * lb;
-        a.l2 = l2;
-        a.c2 = 2.0f * (1.0f - 3.0f * l2);
-        a.lam_b = lb;
-        a.l2b = l2b;
-        a.c2b = 2.0f * (1.0f - 3.0f * l2b);
-    }
+    step_scalars(s, a);
     a.compat = s->compat ? 1 : 0;
     a.nb = s->nb;
     a.e_cell = s->d_e_cell.as<int32_t>();
@@ -1151,6 +1167,7 @@ StepArgs make_args(afx_sim* s)
     a.ring_out = s->d_ring_out.p;
     a.coef_a = s->d_A.p;
     a.coef_b = s->d_B.p;
+    a.coef_m = s->d_M.p;
     a.ncoef = s->has_model ? s->ncoef : 0;
     a.nsp = s->nsp;
     a.sp_cell = s->d_sp_cell.as<int32_t>();

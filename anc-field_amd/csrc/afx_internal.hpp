@@ -71,6 +71,7 @@ struct StepArgs {
     void* ring_out;
     const void* coef_a;      // [nmat][ncoef] of compute type
     const void* coef_b;
+    const void* coef_m;      // [nmat][4] of compute type: l2b/b0, lam_b*a0/b0, a0/(lam_b*b0), 1/a0
     int32_t ncoef;
     // special cells (sources / receivers)
     int32_t nsp;
@@ -146,6 +147,8 @@ hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s);
 hipError_t launch_pml_phys_phi(const PmlArgs& a, hipStream_t s);
 hipError_t launch_pml_compat(const PmlArgs& a, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
+hipError_t launch_mat_consts(Prec prec, const void* A, const void* B, int N, int nmat, double lam_b, double l2b,
+                             void* M, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t s);
 hipError_t launch_to_f32(const void* src, Prec prec, float* dst, int64_t n, double scale, hipStream_t s);
 hipError_t launch_plane_f32(const void* src, Prec prec, float* dst, int nu, int nv, int64_t su, int64_t sv,

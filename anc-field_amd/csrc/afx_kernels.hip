@@ -323,35 +323,63 @@ __device__ __forceinline__ C leapfrog_value(const S* __restrict__ cur, const S* 
     return v - ld<S, C>(prev, i);
 }
 
+// Per-material quotients of the boundary update that depend on the material
+// and the boundary scalars only, written by k_mat_consts with the same
+// operations bnd_value used to evaluate per entry (so the results are
+// bit-identical): {l2b / b0, l * a0 / b0, a0 / (l * b0), 1 / a0}.
+constexpr int kMatConsts = 4;
+
+template <typename C>
+__global__ void k_mat_consts(const C* __restrict__ A, const C* __restrict__ B, int N, int nmat, C l, C l2,
+                             C* __restrict__ M)
+{
+    const int mat = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (mat >= nmat) return;
+    const C a0 = A[(int64_t)mat * N], b0 = B[(int64_t)mat * N];
+    C* mk = M + (int64_t)kMatConsts * mat;
+    mk[0] = l2 / b0;
+    mk[1] = l * a0 / b0;
+    mk[2] = a0 / (l * b0);
+    mk[3] = (C)1.0 / a0;
+}
+
 // New pressure of boundary entry e (rounded to storage precision) with its
 // IIR rings advanced.  Mirrored neighbour sum: axes x, y, z in order; a
-// solid face replaces its pair by 2*opposite (lrs_1.cl:135-175).
+// solid face replaces its pair by 2*opposite (lrs_1.cl:135-175).  ixb is the
+// step's ring index, below N (ring_index).
 template <typename S, typename C>
 __device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* __restrict__ cur,
                        const S* __restrict__ prev, C* __restrict__ rin, C* __restrict__ rout, int64_t rs,
-                       const C* __restrict__ A, const C* __restrict__ B, int N, unsigned ixb, int compat,
-                       int64_t sy, int64_t sz, const BndScalars<C>& k)
+                       const C* __restrict__ A, const C* __restrict__ B, const C* __restrict__ M, int N,
+                       unsigned ixb, int compat, int64_t sy, int64_t sz, const BndScalars<C>& k)
 {
     const uint32_t m = info & kInfoMask;
     const int K = (int)((info >> kInfoKShift) & 3u);
     const unsigned sf = (compat && K == 3) ? (unsigned)N / 2u : (unsigned)N;  // lrs_3.cl:70
-    C a0[3], b0[3], g[3];
+    C b0[3], g[3], mq[3][kMatConsts];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (q >= K) break;
         const int mat = (int)((mats >> (10 * q)) & 1023u);
         const C* ak = A + mat * N;
         const C* bk = B + mat * N;
-        a0[q] = ak[0];
         b0[q] = bk[0];
+#pragma unroll
+        for (int u = 0; u < kMatConsts; ++u) mq[q][u] = M[kMatConsts * mat + u];
         C gn = 0;
         for (int ii = 1; ii < N; ++ii) {
-            const unsigned slot = ((ixb + (unsigned)ii) % sf) + (unsigned)q * sf;
+            // (ixb + ii) mod sf; ixb + ii < 2N, so one subtraction when sf == N
+            unsigned t = ixb + (unsigned)ii;
+            if (sf == (unsigned)N)
+                t = t >= sf ? t - sf : t;
+            else
+                t = t % sf;
+            const unsigned slot = t + (unsigned)q * sf;
             gn += bk[ii] * rin[slot * rs + e] - ak[ii] * rout[slot * rs + e];
         }
         g[q] = gn;
     }
-    const C l = k.lam_b, l2 = k.l2b;
+    const C l2 = k.l2b;
     const C pc = ld<S, C>(cur, i);
     const C pd1 = ld<S, C>(prev, i);
     C v;
@@ -378,20 +406,21 @@ __device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* 
             first = false;
         }
         C num, den;
+        // mq[q][0] = l2b / b0, mq[q][1] = lam_b * a0 / b0
         if (K == 1) {
             num = l2 * acc + k.c2b * pc;
-            num = num + l2 / b0[0] * g[0];
-            num = num + (l * a0[0] / b0[0] - (C)1.0) * pd1;
-            den = (C)1.0 + l * a0[0] / b0[0];
+            num = num + mq[0][0] * g[0];
+            num = num + (mq[0][1] - (C)1.0) * pd1;
+            den = (C)1.0 + mq[0][1];
         } else {
             const int U = (compat && K == 3) ? 2 : K;  // lrs_3.cl:96-174 use materials 0 and 1
             C G = g[0] / b0[0];
-            C Asum = l * a0[0] / b0[0];
-            den = (C)1.0 + l * a0[0] / b0[0];
+            C Asum = mq[0][1];
+            den = (C)1.0 + mq[0][1];
             for (int q = 1; q < U; ++q) {
                 G = G + g[q] / b0[q];
-                Asum = Asum + l * a0[q] / b0[q];
-                den = den + l * a0[q] / b0[q];
+                Asum = Asum + mq[q][1];
+                den = den + mq[q][1];
             }
             num = l2 * acc + l2 * G;
             num = num + k.c2b * pc;
@@ -403,10 +432,11 @@ __device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* 
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (q >= K) break;
-        const C xn = a0[q] / (l * b0[q]) * (v - pd1) - g[q] / b0[q];
+        // mq[q][2] = a0 / (lam_b * b0), mq[q][3] = 1 / a0
+        const C xn = mq[q][2] * (v - pd1) - g[q] / b0[q];
         const unsigned w = ixb + (unsigned)q * sf;
         rin[w * rs + e] = xn;
-        rout[w * rs + e] = (C)1.0 / a0[q] * (b0[q] * xn + g[q]);
+        rout[w * rs + e] = mq[q][3] * (b0[q] * xn + g[q]);
     }
     return v;
 }
@@ -436,6 +466,7 @@ struct BndCtx {
     int64_t rs;
     const C* A;
     const C* B;
+    const C* M;       // per-material quotients (k_mat_consts)
     int N;
     int compat;
     BndScalars<C> k;
@@ -487,7 +518,7 @@ __device__ __forceinline__ void boundary_entry(const BndCtx<S, C>& c, int e, uns
     const RoomDev r = c.rooms[info >> kInfoRoomShift];
     const int64_t i = c.e_cell[e];
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
-    const C v = bnd_value<S, C>(e, i, info, c.e_mat[e], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+    const C v = bnd_value<S, C>(e, i, info, c.e_mat[e], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.M, c.N, ixb,
                                 c.compat, sy, sz, c.k);
     st<S, C>(c.out, i, v);
 }
@@ -501,7 +532,7 @@ __device__ __forceinline__ void boundary_packed(const BndCtx<S, C>& c, int4 en, 
     const RoomDev r = c.rooms[info >> kInfoRoomShift];
     const int64_t i = en.x;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
-    const C v = bnd_value<S, C>(en.w, i, info, (uint32_t)en.z, c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+    const C v = bnd_value<S, C>(en.w, i, info, (uint32_t)en.z, c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.M, c.N, ixb,
                                 c.compat, sy, sz, c.k);
     st<S, C>(c.out, i, v);
 }
@@ -539,7 +570,7 @@ __device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsi
     if (full && b >= 0) {
         const uint32_t info = c.e_info[b];
         const RoomDev r = c.rooms[info >> kInfoRoomShift];
-        v = bnd_value<S, C>(b, i, info, c.e_mat[b], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.N, ixb,
+        v = bnd_value<S, C>(b, i, info, c.e_mat[b], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.M, c.N, ixb,
                             c.compat, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k);
     } else if (full && b == -1) {
         const RoomDev r = c.rooms[c.sp_room[j]];
@@ -736,6 +767,7 @@ __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
     c.rs = a.nb;
     c.A = (const C*)a.coef_a;
     c.B = (const C*)a.coef_b;
+    c.M = (const C*)a.coef_m;
     c.N = a.ncoef;
     c.compat = a.compat;
     c.k.lam_b = (C)a.lam_b;
@@ -1326,6 +1358,20 @@ hipError_t launch_pml_compat(const PmlArgs& a, hipStream_t s)
     case Prec::BF16: return pml_compat_s<__bf16>(a, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_mat_consts(Prec prec, const void* A, const void* B, int N, int nmat, double lam_b, double l2b,
+                             void* M, hipStream_t s)
+{
+    if (nmat <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((nmat + 255) / 256);
+    if (prec == Prec::F64)
+        hipLaunchKernelGGL(k_mat_consts<double>, dim3(blocks), dim3(256), 0, s, (const double*)A, (const double*)B, N,
+                           nmat, lam_b, l2b, (double*)M);
+    else
+        hipLaunchKernelGGL(k_mat_consts<float>, dim3(blocks), dim3(256), 0, s, (const float*)A, (const float*)B, N,
+                           nmat, (float)lam_b, (float)l2b, (float*)M);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s)
