@@ -100,6 +100,55 @@ void orc_set_threads(int n)
 #endif
 }
 
+/* Round to nearest, ties to even, into binary16 and back to float.
+ * Normal range (|x| >= 2^-14): the 13 significand bits binary16 lacks are
+ * rounded off by adding 0x0FFF plus the lowest kept bit and truncating, so
+ * a carry out of the significand moves into the exponent; results of 65520
+ * and above (beyond the largest finite 65504) become infinity.  Subnormal
+ * range: the quantum is 2^-24, the ulp of 0.75f, so (|x| + 0.75f) - 0.75f
+ * rounds |x| to it under the default rounding mode. */
+static float orc_round_f16(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint32_t sign = u & 0x80000000u, a = u & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) return x; /* inf, nan */
+    uint32_t r;
+    if (a >= 0x38800000u) {
+        r = (a + 0x0FFFu + ((a >> 13) & 1u)) & ~0x1FFFu;
+        if (r >= 0x47800000u) r = 0x7F800000u;
+    } else {
+        float m;
+        memcpy(&m, &a, 4);
+        volatile float t = m + 0.75f;
+        m = t - 0.75f;
+        memcpy(&r, &m, 4);
+    }
+    r |= sign;
+    float y;
+    memcpy(&y, &r, 4);
+    return y;
+}
+
+/* Round to nearest, ties to even, into bfloat16 (the upper half of a float)
+ * and back: the same carry rounding on the low 16 bits; a carry out of the
+ * largest finite value gives infinity.  NaN is returned unchanged. */
+static float orc_round_bf16(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return x;
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    float y;
+    memcpy(&y, &u, 4);
+    return y;
+}
+
+float orc_round_store(float x, int32_t mode)
+{
+    return mode == 1 ? orc_round_f16(x) : (mode == 2 ? orc_round_bf16(x) : x);
+}
+
 #define REAL float
 #define SFX _f32
 #include "orc_impl.h"

@@ -31,6 +31,10 @@
  *                    zeta == 0; absorption against a rigid/Dirichlet wall.
  *   source / mic     source.cl:32-40, mic.cl:32-41, simulation.hpp:384-392,
  *                    positions simulation.hpp:322,338.
+ *   storage rounding no reference counterpart (FP16/BF16 field storage).
+ *                    orc_round_store is pinned against numpy's float16 and
+ *                    torch's bfloat16 conversions over normal, subnormal,
+ *                    tie and overflow inputs (tests).
  */
 #ifndef AFX_ORACLE_H
 #define AFX_ORACLE_H
@@ -73,6 +77,12 @@ typedef struct {
     const double* ksi_h[3];   /* physical: zeta at +1/2 points */
     const int32_t* pml_boxes; /* physical: [nbox][6] x0,x1,y0,y1,z0,z1 (half-open), disjoint */
     int32_t pml_nbox;
+    /* reduced-precision field storage (f32 instantiation only): 0 none,
+     * 1 IEEE binary16, 2 bfloat16.  Fields hold pressure / scale (a power of
+     * two, <= 0 meaning 1); every pressure write rounds to the storage type,
+     * sources enter as sample * (1/scale), traces are value * scale. */
+    int32_t store;
+    double scale;
 } orc_problem;
 
 /* Boundary classification restating lrs-data.hpp.  Writes up to `cap`
@@ -103,6 +113,10 @@ int64_t      orc_nbound_f64(const orc_sim_f64* s);
 void         orc_rings_f64(const orc_sim_f64* s, double* in, double* out);
 
 void         orc_set_threads(int n);
+
+/* x rounded to nearest-even in the storage type `mode` (1 binary16,
+ * 2 bfloat16) and widened back to float; mode 0 returns x. */
+float        orc_round_store(float x, int32_t mode);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,15 @@ struct SIM {
     REAL* ksi[3];
     REAL* ksi_h[3];
     REAL* snap;          /* compat pml: unused scratch */
+    int32_t store;       /* storage rounding mode (orc_problem.store) */
+    REAL scale, inv_scale;
 };
+
+/* a pressure value as the field storage holds it */
+static inline REAL FN(rnd)(const struct SIM* s, REAL v)
+{
+    return s->store ? (REAL)orc_round_store((float)v, s->store) : v;
+}
 
 static inline int64_t FN(ix)(const struct SIM* s, int64_t x, int64_t y, int64_t z)
 {
@@ -62,6 +70,10 @@ struct SIM* FN(orc_create)(const orc_problem* pb)
     /* simulation.hpp:368-369 passes lambda2_ where the kernels expect lambda */
     s->lam_b = pb->compat ? s->lambda2 : (dt * c / dx);
     s->c2i = (REAL)2.0 * ((REAL)1.0 - (REAL)3.0 * s->lambda2);
+    if (pb->store != 0 && sizeof(REAL) != sizeof(float)) { FN(orc_destroy)(s); return NULL; }
+    s->store = pb->store;
+    s->scale = (REAL)(pb->scale > 0 ? pb->scale : 1.0);
+    s->inv_scale = (REAL)(1.0 / (pb->scale > 0 ? pb->scale : 1.0));
     int64_t nab = (int64_t)pb->nmat * pb->ncoef;
     s->a = (REAL*)calloc((size_t)(nab > 0 ? nab : 1), sizeof(REAL));
     s->b = (REAL*)calloc((size_t)(nab > 0 ? nab : 1), sizeof(REAL));
@@ -135,7 +147,7 @@ static void FN(leapfrog)(struct SIM* s, REAL* p, const REAL* p_d1, const REAL* p
                 sum = sum + p_d1[i + sz];
                 sum = sum + p_d1[i - sz];
                 REAL v = l2 * sum + c2 * p_d1[i];
-                p[i] = v - p_d2[i];
+                p[i] = FN(rnd)(s, v - p_d2[i]);
             }
         }
 }
@@ -227,7 +239,7 @@ static void FN(lrs_cell)(struct SIM* s, int64_t e, REAL* p_p1, const REAL* p, co
             num = num + cb * p[i];
             num = num + (A - (REAL)1.0) * p_d1[i];
         }
-        p_p1[i] = num / den;
+        p_p1[i] = FN(rnd)(s, num / den);
     }
     const REAL pp = p_p1[i];
     for (int k = 0; k < K; ++k) {
@@ -284,7 +296,7 @@ static void FN(pml_compat)(struct SIM* s, REAL* p, const REAL* pm1, const REAL* 
                 REAL t5 = (kx[x] * ky[y] + ky[y] * kz[z] + kz[z] * kx[x]) * pm1[i];
                 REAL t6 = (kx[x] + ky[y] + kz[z]) / ((REAL)2.0 * dt) * pm2[i];
                 REAL num = t1 + t2 - t3 - t4 - t5 + t6;
-                p[i] = num / ((REAL)1.0 / (dt * dt) + (kx[x] + ky[y] + kz[z]) / ((REAL)2.0 * dt));
+                p[i] = FN(rnd)(s, num / ((REAL)1.0 / (dt * dt) + (kx[x] + ky[y] + kz[z]) / ((REAL)2.0 * dt)));
             }
     #pragma omp parallel for schedule(static)
     for (int64_t z = 2; z < nz - 2; ++z)
@@ -353,7 +365,7 @@ static void FN(pml_phys_p)(struct SIM* s, REAL* p1, const REAL* p, const REAL* p
                     num = num - (dt * dt) * s2 * p[i];
                     num = num + (REAL)2.0 * p[i] - pm[i];
                     num = num + (REAL)0.5 * dt * s1 * pm[i];
-                    p1[i] = num / ((REAL)1.0 + (REAL)0.5 * dt * s1);
+                    p1[i] = FN(rnd)(s, num / ((REAL)1.0 + (REAL)0.5 * dt * s1));
                     psi[i] = psn;
                 }
     }
@@ -426,12 +438,19 @@ int FN(orc_run)(struct SIM* s, int64_t steps, REAL* traces)
         for (int i = 0; i < pb->nsrc; ++i) {
             const int32_t* q = pb->src_cell + 3 * i;
             REAL v = t < pb->src_len ? s->src[i * pb->src_len + t] : (REAL)0.0;
-            p_p1[FN(ix)(s, q[0], q[1], q[2])] += v;
+            p_p1[FN(ix)(s, q[0], q[1], q[2])] += v * s->inv_scale;
         }
+        /* all of a cell's sources are added before it is stored */
+        if (s->store)
+            for (int i = 0; i < pb->nsrc; ++i) {
+                const int32_t* q = pb->src_cell + 3 * i;
+                const int64_t j = FN(ix)(s, q[0], q[1], q[2]);
+                p_p1[j] = FN(rnd)(s, p_p1[j]);
+            }
         if (traces)
             for (int i = 0; i < pb->nmic; ++i) {
                 const int32_t* q = pb->mic_cell + 3 * i;
-                traces[(int64_t)i * steps + it] = p_p1[FN(ix)(s, q[0], q[1], q[2])];
+                traces[(int64_t)i * steps + it] = p_p1[FN(ix)(s, q[0], q[1], q[2])] * s->scale;
             }
         if (pb->pml_mode == 2) FN(pml_phys_phi)(s, p_p1, p);
         if (pb->pml_mode == 1 && it % 8000 == 0) {

@@ -4,6 +4,7 @@ Mirrors the library's inputs (voxels, materials, sources, receivers, PML)
 so a test can run the same workload through both and compare.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -29,6 +30,7 @@ class Problem(C.Structure):
         ("pml_mode", C.c_int32), ("pml_width", C.c_int32),
         ("ksi", _f64p * 3), ("ksi_h", _f64p * 3),
         ("pml_boxes", _i32p), ("pml_nbox", C.c_int32),
+        ("store", C.c_int32), ("scale", C.c_double),
     ]
 
 
@@ -51,10 +53,28 @@ def _load():
     lib.orc_classify.restype = C.c_int64
     lib.orc_classify.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.c_int64]
     lib.orc_set_threads.argtypes = [C.c_int]
+    lib.orc_round_store.restype = C.c_float
+    lib.orc_round_store.argtypes = [C.c_float, C.c_int32]
     return lib
 
 
 lib = _load()
+
+STORE_MODES = {None: 0, "fp16": 1, "bf16": 2}
+
+
+def round_store(x, storage):
+    """float32 values rounded to the storage type and widened back."""
+    mode = STORE_MODES[storage]
+    x = np.asarray(x, np.float32)
+    return np.array([lib.orc_round_store(float(v), mode) for v in x.ravel()], np.float32).reshape(x.shape)
+
+
+def storage_scale(src):
+    """The power-of-two scale the library picks for FP16/BF16 storage: the
+    smallest 2^k at or above the sources' peak amplitude."""
+    mx = float(np.max(np.abs(np.asarray(src, np.float32)))) if np.size(src) else 0.0
+    return math.ldexp(1.0, math.ceil(math.log2(mx))) if mx > 0 else 1.0
 
 
 def classify(vox):
@@ -157,7 +177,8 @@ class Oracle:
     """One oracle simulation; precision 'f32' or 'f64'."""
 
     def __init__(self, vox, a, b, c, dx, dt, compat, src_cells=(), src=None, mic_cells=(), precision="f32",
-                 pml_mode=0, pml_width=0, pml_faces=0, pml_profile=None, dims=None, threads=8):
+                 pml_mode=0, pml_width=0, pml_faces=0, pml_profile=None, dims=None, threads=8,
+                 storage=None, scale=None):
         self.sfx = precision
         self.T = np.float32 if precision == "f32" else np.float64
         self._keep = []
@@ -211,6 +232,11 @@ class Oracle:
             pb.pml_boxes = boxes.ctypes.data_as(_i32p)
             pb.pml_nbox = boxes.shape[0]
             self._keep += zc + zh + [boxes]
+        # FP16/BF16 field storage: f32 arithmetic, values rounded on every
+        # store, fields in units of pressure / scale
+        pb.store = STORE_MODES[storage]
+        self.scale = float(scale) if scale else (storage_scale(src) if storage else 1.0)
+        pb.scale = self.scale
         self._pb = pb
         lib.orc_set_threads(threads)
         self._h = getattr(lib, f"orc_create_{self.sfx}")(C.byref(pb))
@@ -232,7 +258,7 @@ class Oracle:
         out = np.zeros(self.dims, self.T)
         ptr = out.ctypes.data_as(C.POINTER(C.c_float if self.T == np.float32 else C.c_double))
         getattr(lib, f"orc_field_{self.sfx}")(self._h, which, ptr)
-        return out
+        return out * self.T(self.scale)  # pressure; the scale is a power of two
 
     def nbound(self):
         return getattr(lib, f"orc_nbound_{self.sfx}")(self._h)
