@@ -2,8 +2,8 @@
 
 Each case draws a room (size, furniture boxes, wall materials with 1 to 4 IIR
 coefficients), physical or compat mode, a PML (physical: random faces and
-width; compat: the reference's PML or none), FP32 or FP64, a tile shape and
-one or two steps per pass.  Sources and receivers include interior cells and
+width; compat: the reference's PML or none), FP32 or FP64 (and each draw
+again with FP16 or BF16 storage), a tile shape and one or two steps per pass.  Sources and receivers include interior cells and
 a cell on the grid's edge plane.  The draws are fixed by the seed, so a
 failure names a reproducible configuration.
 """
@@ -73,7 +73,18 @@ def _draw(seed):
 
 @pytest.mark.parametrize("seed", range(8))
 def test_random_configuration_matches_oracle(seed):
+    _check(_draw(seed), seed)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_configuration_reduced_storage_matches_oracle(seed):
+    """The same draws with FP16 (even seeds) or BF16 (odd seeds) storage."""
     d = _draw(seed)
+    d["precision"] = "fp16" if seed % 2 == 0 else "bf16"
+    _check(d, seed)
+
+
+def _check(d, seed):
     v = d["v"]
     nz, ny, nx = v.shape
     pml = d["pml"]
@@ -103,7 +114,8 @@ def test_random_configuration_matches_oracle(seed):
     s.close()
     o = orc.Oracle(v, d["a"], d["b"], c, dx, dt, compat, d["sc"], src, d["mc"],
                    "f64" if d["precision"] == "fp64" else "f32", pml_mode,
-                   pml[0] if pml else 0, pml[1] if pml else 0, pml[2] if pml else None)
+                   pml[0] if pml else 0, pml[1] if pml else 0, pml[2] if pml else None,
+                   storage=d["precision"] if d["precision"] in ("fp16", "bf16") else None)
     o.run(steps)
     want, wfield = o.receivers(), o.field(0)
     assert np.array_equal(got, np.asarray(want).astype(got.dtype)), "traces differ"
