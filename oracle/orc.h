@@ -22,9 +22,13 @@
  *                    reflection R=(1-beta)/(1+beta) (physical), IIR ring
  *                    recursion against scipy.signal.lfilter (tests).
  *   pml (compat)     pml.cl:74-265, simulation.hpp:233-277, :373-382,
- *                    :411-430.  Pinned only at the shipped state (kappa == 0,
- *                    where it must reduce to the plain leapfrog); with a
- *                    non-zero profile: parity unpinned.
+ *                    :411-430.  Pinned at the shipped state (kappa == 0,
+ *                    where it must reduce to the plain leapfrog) and, with
+ *                    the intended profile, by its absorption, the weak echo
+ *                    of its graded inner face and the untouched direct
+ *                    sound (tests).  Parity unpinned for the corner
+ *                    kappa_x kappa_y kappa_z psi term and the phi-update
+ *                    quirks, which those properties do not distinguish.
  *   pml (physical)   no reference counterpart (Grote & Sim 2010 PML for the
  *                    scalar wave equation, the model pml.cl/sym_pml.py set
  *                    out to discretise).  Pinned: reduces to leapfrog when

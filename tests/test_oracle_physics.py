@@ -179,6 +179,40 @@ def test_compat_pml_zero_profile_is_leapfrog():
     assert not np.array_equal(x, y)  # computed by a different formula
 
 
+def test_compat_pml_absorbs_with_shipped_profile():
+    """pml.cl with the intended kappa profile (ksi_profile.dat, 8000 (k/16 -
+    sin(2 pi k/16) / 2 pi), reversed at both ends by SetPml) in a 64^3 box:
+
+    - the direct sound (steps 0-80) carries the same energy as without the
+      layer, to 1e-3 (kappa and phi are zero away from the edges);
+    - the echo from the layer's inner face (steps 80-160) stays below 1e-2 of
+      the direct sound's energy: the profile rises from zero there.  Left
+      unreversed, it starts at its maximum and the jump reflects 4.6e-2;
+    - the late field (steps 450-700) holds less than 1e-4 of the energy of
+      the same box with pressure-release edges.  A flipped sign of the
+      kappa/(2 dt) pm2 term gives 1.4, kappa missing from the denominator
+      blows up.
+
+    The corner term (kappa_x kappa_y kappa_z psi) and the reference's phi
+    update quirks are not distinguished by these properties."""
+    n, steps = 64, 700
+    dt = 3.125e-05
+    sig = rooms.ricker(steps, dt, 1000.0)
+    src, mics = [[32, 32, 32]], [[36, 32, 32], [32, 40, 30]]
+    k = np.arange(16)
+    shipped = (8000 * (k / 16 - np.sin(2 * np.pi * k / 16) / (2 * np.pi))).astype(np.float32)
+    bare = orc.Oracle(None, None, None, 340.0, 0.02, dt, True, src, sig[None], mics, precision="f64",
+                      dims=(n, n, n)).run(steps)
+    lay = orc.Oracle(None, None, None, 340.0, 0.02, dt, True, src, sig[None], mics, precision="f64",
+                     dims=(n, n, n), pml_mode=1, pml_width=16, pml_profile=shipped).run(steps)
+    assert np.all(np.isfinite(lay))
+    e = lambda t, lo, hi: float(np.sum(t[:, lo:hi] ** 2))
+    direct = e(bare, 0, 80)
+    assert abs(e(lay, 0, 80) / direct - 1.0) < 1e-3
+    assert e(lay, 80, 160) < 1e-2 * direct, e(lay, 80, 160) / direct
+    assert e(lay, 450, 700) < 1e-4 * e(bare, 450, 700), e(lay, 450, 700) / e(bare, 450, 700)
+
+
 def test_physical_pml_absorbs():
     """Energy left in a box after a pulse hits a PML face is far below what
     the same face reflects as a pressure-release (p=0) edge."""
