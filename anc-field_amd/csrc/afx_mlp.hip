@@ -17,6 +17,8 @@
 //                      OUT     y = C + b (FP32)
 //                      LOSS    y = C + b; g = 2 w (y - t) -> BF16 both layouts; per-wave loss partial
 //                      BACK    g = C * silu'(z_prev) -> BF16 both layouts
+//   k_gemm128<EPI>   HIDDEN and OUT for inference chunks: 128x128 tiles through
+//                    LDS, k-slices visited in turn (same bits as k_gemm)
 //   k_gemm64<EPI>    HIDDEN and OUT for inference chunks: 64x64 tiles through LDS,
 //                    the same bits as k_gemm
 //   k_grads          all weight gradients of a step, dW_l = dY_l^T . X_l (A = dY_l^T
@@ -350,6 +352,111 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
         }
 }
 
+// Inference GEMM for large M on 128x128 output tiles (HIDDEN and OUT
+// epilogues): four waves, each a 64x64 quadrant of 4x4 MFMA blocks, operands
+// staged through LDS in k-steps of 32 (double buffered, one barrier per
+// step).  The bits are k_gemm's: every MFMA sees the same operands per lane,
+// and each output is the sum of four k-slice partials (slice = k-step mod 4)
+// in k_gemm's order.  The k-steps are visited slice by slice (0, 4, 8, ...,
+// then 1, 5, 9, ...), so only the current slice's partials and the running
+// sum of the finished slices are held: ((p0 + p1) + p2) + p3 as in k_gemm64,
+// with half its accumulator registers per output.  Needs M, N and K
+// multiples of 128.
+template <int E>
+__global__ __launch_bounds__(256) void k_gemm128(Gemm g)
+{
+    constexpr int kPad = 40;  // LDS row stride in bf16 (32 + 8)
+    __shared__ bf16 As[2][128][kPad];
+    __shared__ bf16 Bs[2][128][kPad];
+    const int tiles_n = g.N >> 7;
+    const int m0 = (int)(blockIdx.x / tiles_n) << 7;
+    const int n0 = (int)(blockIdx.x % tiles_n) << 7;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    const int lr = tid >> 2, lc = (tid & 3) << 3;  // rows lr and lr + 64, one 16-byte piece each
+    const bf16* ga0 = g.A + (size_t)(m0 + lr) * g.lda + lc;
+    const bf16* ga1 = ga0 + (size_t)64 * g.lda;
+    const bf16* gb0 = g.B + (size_t)(n0 + lr) * g.ldb + lc;
+    const bf16* gb1 = gb0 + (size_t)64 * g.ldb;
+    const int nk = g.K >> 5, ns = nk >> 2;  // k-steps; k-steps per slice
+    // visit j takes k-step (j / ns) + 4 (j % ns): slice j / ns, its (j % ns)-th step
+    bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(ga0);
+    bf16x8 ra1 = *reinterpret_cast<const bf16x8*>(ga1);
+    bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(gb0);
+    bf16x8 rb1 = *reinterpret_cast<const bf16x8*>(gb1);
+    *reinterpret_cast<bf16x8*>(&As[0][lr][lc]) = ra0;
+    *reinterpret_cast<bf16x8*>(&As[0][lr + 64][lc]) = ra1;
+    *reinterpret_cast<bf16x8*>(&Bs[0][lr][lc]) = rb0;
+    *reinterpret_cast<bf16x8*>(&Bs[0][lr + 64][lc]) = rb1;
+    __syncthreads();
+    const int r = lane & 15, kq = (lane >> 4) << 3;
+    f32x4 sum[4][4], p[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sum[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    for (int v = 0; v < nk; ++v) {
+        const int cur = v & 1;
+        const bool more = v + 1 < nk;
+        if (more) {
+            const int kn = 32 * ((v + 1) / ns + 4 * ((v + 1) % ns));
+            ra0 = *reinterpret_cast<const bf16x8*>(ga0 + kn);
+            ra1 = *reinterpret_cast<const bf16x8*>(ga1 + kn);
+            rb0 = *reinterpret_cast<const bf16x8*>(gb0 + kn);
+            rb1 = *reinterpret_cast<const bf16x8*>(gb1 + kn);
+        }
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][64 * wi + 16 * i + r][kq]);
+            b[i] = *reinterpret_cast<const bf16x8*>(&Bs[cur][64 * wj + 16 * i + r][kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
+        if (v % ns == ns - 1) {  // slice v / ns complete
+            const bool first = v < ns;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    sum[i][j] = first ? p[i][j] : sum[i][j] + p[i][j];
+                    p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
+        }
+        if (more) {
+            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr][lc]) = ra0;
+            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr + 64][lc]) = ra1;
+            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr][lc]) = rb0;
+            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr + 64][lc]) = rb1;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + 64 * wj + 16 * j + r;
+            const int mb = m0 + 64 * wi + 16 * i + ((lane >> 4) << 2);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v = sum[i][j][e];
+                const int m = mb + e;
+                if (E == EPI_HIDDEN) {
+                    const float z = v + g.bias[n];
+                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
+                    g.act[(size_t)m * g.ldact + n] = (bf16)silu(z);
+                } else {
+                    g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+                }
+            }
+        }
+}
+
 // Weight-gradient tile `bid` of the grouped problems (with AdamW fused when
 // ad.on).
 __device__ inline void grads_tile(const GradTable& t, AdamArgs ad, const StepRef& sr, int bid,
@@ -593,7 +700,18 @@ namespace {
 int gemm_launch(afx_mlp* s, int epi, Gemm g)
 {
     if (g.M % 32 || g.N % 32 || g.K % 32) return err(AFX_E_ARG, "gemm %dx%dx%d not a multiple of 32", g.M, g.N, g.K);
-    // large-M inference GEMMs on 64x64 tiles (same bits; see k_gemm64)
+    // large-M inference GEMMs on 128x128 or 64x64 tiles (same bits; see
+    // k_gemm128 and k_gemm64)
+    if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 128 == 0 && g.N % 128 == 0 &&
+        g.K % 128 == 0) {
+        const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(256);
+        if (epi == EPI_HIDDEN)
+            k_gemm128<EPI_HIDDEN><<<grid128, block128, 0, s->stream>>>(g);
+        else
+            k_gemm128<EPI_OUT><<<grid128, block128, 0, s->stream>>>(g);
+        HIP_CHECK(hipGetLastError());
+        return AFX_OK;
+    }
     if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 64 == 0 && g.N % 64 == 0 &&
         g.K % 128 == 0) {
         const dim3 grid64((unsigned)((g.M / 64) * (g.N / 64))), block64(256);
@@ -787,7 +905,7 @@ void init_params(const afx_mlp* s, std::vector<float>& p)
 
 namespace {
 
-constexpr int kInferRows = 8192;
+constexpr int kInferRows = 32768;
 
 int ensure_infer(afx_mlp* s)
 {

@@ -164,7 +164,7 @@ def surrogate_leg(train_steps, eager=False):
         e_tr = sur.validate(ps, pr, tr_ir)
         fwd = sur.net.stats()
         # inference with queries and responses resident in HBM: 65 536 rows
-        # (8 chunks of 8192) through afx_mlp_forward_device, best of three
+        # (2 chunks of 32 768) through afx_mlp_forward_device, best of three
         from anc_field_py import _afx
         rows_dev = 65536
         xr = np.random.default_rng(0).uniform(0, 1, (rows_dev, 6)).astype(np.float32)

@@ -140,13 +140,14 @@ def test_mlp_graph_replay_matches_eager_training():
 
 
 def test_mlp_forward_chunking_and_device_path():
-    """Inference chunks of up to 8192 rows: 10 000 rows (a full chunk written
-    in place on the device path, then a padded 1808-row tail) give the same
-    bits as 300-row pieces, from host and from device memory."""
-    print("\n[case] mlp forward 10 000 rows: host, device, 300-row pieces", flush=True)
+    """Inference chunks of up to 32 768 rows: 40 000 rows (a full chunk on
+    128x128 tiles, written in place on the device path, then a 7232-row tail
+    on 64x64 tiles) give the same bits as 300-row pieces (split-K 32x32
+    kernels), from host and from device memory."""
+    print("\n[case] mlp forward 40 000 rows: host, device, 300-row pieces", flush=True)
     from anc_field_py import _afx
     rng = np.random.default_rng(9)
-    n = 10000
+    n = 40000
     x = rng.uniform(0, 1, (n, 6)).astype(np.float32)
     with _afx.Mlp(seed=13) as m:
         y = m.forward(x)
@@ -163,10 +164,11 @@ def test_mlp_forward_chunking_and_device_path():
 
 
 def test_mlp_large_m_gemm_matches_training_kernels():
-    """Forward passes of 1024 rows or more run on 64x64 tiles staged through
+    """Forward passes of 1024 rows or more run on 128x128 tiles (row counts
+    that are multiples of 128) or 64x64 tiles (multiples of 64) staged through
     LDS; their outputs equal, bit for bit, the outputs the training path's
     split-K 32x32 kernels give for the same rows."""
-    print("\n[case] mlp 64x64 inference tiles vs split-K training kernels", flush=True)
+    print("\n[case] mlp 128x128 and 64x64 inference tiles vs split-K training kernels", flush=True)
     from anc_field_py import _afx
     rng = np.random.default_rng(10)
     x = rng.uniform(0, 1, (2048, 6)).astype(np.float32)
@@ -174,5 +176,8 @@ def test_mlp_large_m_gemm_matches_training_kernels():
     w = np.full(256, 1.0 / 256, np.float32)
     with _afx.Mlp(seed=21) as m:
         _, _, y_train = m.gradients(x[:256], t, w)
-        y_inf = m.forward(x)
-    assert np.array_equal(y_inf[:256], y_train)
+        y_128 = m.forward(x)          # 2048 rows: 128x128 tiles
+        y_64 = m.forward(x[:1088])    # 1088 rows: 64x64 tiles
+    assert np.array_equal(y_128[:256], y_train)
+    assert np.array_equal(y_64[:256], y_train)
+    assert np.array_equal(y_64, y_128[:1088])
