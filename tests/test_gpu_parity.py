@@ -222,6 +222,40 @@ def test_batched_rooms_match_single():
         k += len(w.mic_cells)
 
 
+@pytest.mark.parametrize("precision,tblock", [("fp32", 1), ("fp32", 2), ("fp16", 1)])
+def test_batched_rooms_match_oracle(precision, tblock):
+    """Each room of one handle, all with a physical PML on x+ and z-, against
+    its own oracle run, bit for bit."""
+    _case(f"batched rooms vs the oracle per room, {precision}, {tblock} step(s) per pass")
+    ws = rooms.batch_rooms(seed=11, nrooms=3, steps=130, lo=(70, 24, 24), hi=(140, 40, 40), nmic=3)
+    faces = afx.FACE_XP | afx.FACE_ZM
+    width = 5
+    prof = (3e4 * ((np.arange(width) + 0.5) / width) ** 3).astype(np.float32)
+    s = afx.Sim(c=ws[0].c, dx=ws[0].dx, dt=ws[0].dt, res=ws[0].dx, compat=False, precision=precision)
+    s.set_voxels(ws[0].vox)
+    s.set_materials(ws[0].a, ws[0].b)
+    s.set_pml(width, faces, prof)
+    s.set_time_block(tblock)
+    ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
+    for rid, w in zip(ids, ws):
+        for cell, sig in zip(w.src_cells, w.src):
+            s.add_source_cell(cell, sig, room=rid)
+    for rid, w in zip(ids, ws):
+        for cell in w.mic_cells:
+            s.add_receiver_cell(cell, room=rid)
+    s.run(ws[0].steps)
+    tr = s.receivers()
+    s.close()
+    k = 0
+    for w in ws:
+        o = orc.Oracle(w.vox, w.a, w.b, w.c, w.dx, w.dt, False, w.src_cells, w.src, w.mic_cells, "f32", 2, width,
+                       faces, prof, storage=precision if precision != "fp32" else None)
+        want = o.run(w.steps)
+        _assert_bitexact(tr[k:k + len(w.mic_cells)], want, w.name)
+        assert np.abs(want).max() > 0
+        k += len(w.mic_cells)
+
+
 @pytest.mark.parametrize("fused", [True, False])
 def test_batched_rooms_with_pml_match_single(fused):
     """Every room of a handle gets the PML; each room equals its own run, with
