@@ -353,9 +353,9 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
 }
 
 // Inference GEMM for large M on 128x128 output tiles (HIDDEN and OUT
-// epilogues): four waves, each a 64x64 quadrant of 4x4 MFMA blocks, operands
+// epilogues): eight waves, each a 64x32 part of 4x2 MFMA blocks, operands
 // staged through LDS in k-steps of 32 (double buffered, one barrier per
-// step).  The bits are k_gemm's: every MFMA sees the same operands per lane,
+// step); 101 VGPRs, so four waves per SIMD hide the operand loads.  The bits are k_gemm's: every MFMA sees the same operands per lane,
 // and each output is the sum of four k-slice partials (slice = k-step mod 4)
 // in k_gemm's order.  The k-steps are visited slice by slice (0, 4, 8, ...,
 // then 1, 5, 9, ...), so only the current slice's partials and the running
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
 // with half its accumulator registers per output.  Needs M, N and K
 // multiples of 128.
 template <int E>
-__global__ __launch_bounds__(256) void k_gemm128(Gemm g)
+__global__ __launch_bounds__(512) void k_gemm128(Gemm g)
 {
     constexpr int kPad = 40;  // LDS row stride in bf16 (32 + 8)
     __shared__ bf16 As[2][128][kPad];
@@ -372,29 +372,22 @@ __global__ __launch_bounds__(256) void k_gemm128(Gemm g)
     const int m0 = (int)(blockIdx.x / tiles_n) << 7;
     const int n0 = (int)(blockIdx.x % tiles_n) << 7;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = wave >> 1, wj = wave & 1;
-    const int lr = tid >> 2, lc = (tid & 3) << 3;  // rows lr and lr + 64, one 16-byte piece each
-    const bf16* ga0 = g.A + (size_t)(m0 + lr) * g.lda + lc;
-    const bf16* ga1 = ga0 + (size_t)64 * g.lda;
-    const bf16* gb0 = g.B + (size_t)(n0 + lr) * g.ldb + lc;
-    const bf16* gb1 = gb0 + (size_t)64 * g.ldb;
-    const int nk = g.K >> 5, ns = nk >> 2;  // k-steps; k-steps per slice
-    // visit j takes k-step (j / ns) + 4 (j % ns): slice j / ns, its (j % ns)-th step
-    bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(ga0);
-    bf16x8 ra1 = *reinterpret_cast<const bf16x8*>(ga1);
-    bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(gb0);
-    bf16x8 rb1 = *reinterpret_cast<const bf16x8*>(gb1);
-    *reinterpret_cast<bf16x8*>(&As[0][lr][lc]) = ra0;
-    *reinterpret_cast<bf16x8*>(&As[0][lr + 64][lc]) = ra1;
-    *reinterpret_cast<bf16x8*>(&Bs[0][lr][lc]) = rb0;
-    *reinterpret_cast<bf16x8*>(&Bs[0][lr + 64][lc]) = rb1;
+    const int wi = wave >> 2, wj = wave & 3;  // 64-row half, 32-column quarter
+    const int lr = tid >> 2, lc = (tid & 3) << 3;  // row lr, one 16-byte piece of each k-step
+    const bf16* ga = g.A + (size_t)(m0 + lr) * g.lda + lc;
+    const bf16* gb = g.B + (size_t)(n0 + lr) * g.ldb + lc;
+    const int nk = g.K >> 5, ns = nk >> 2;
+    bf16x8 ra = *reinterpret_cast<const bf16x8*>(ga);
+    bf16x8 rb = *reinterpret_cast<const bf16x8*>(gb);
+    *reinterpret_cast<bf16x8*>(&As[0][lr][lc]) = ra;
+    *reinterpret_cast<bf16x8*>(&Bs[0][lr][lc]) = rb;
     __syncthreads();
     const int r = lane & 15, kq = (lane >> 4) << 3;
-    f32x4 sum[4][4], p[4][4];
+    f32x4 sum[4][2], p[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 2; ++j) {
             sum[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
@@ -403,44 +396,39 @@ __global__ __launch_bounds__(256) void k_gemm128(Gemm g)
         const bool more = v + 1 < nk;
         if (more) {
             const int kn = 32 * ((v + 1) / ns + 4 * ((v + 1) % ns));
-            ra0 = *reinterpret_cast<const bf16x8*>(ga0 + kn);
-            ra1 = *reinterpret_cast<const bf16x8*>(ga1 + kn);
-            rb0 = *reinterpret_cast<const bf16x8*>(gb0 + kn);
-            rb1 = *reinterpret_cast<const bf16x8*>(gb1 + kn);
+            ra = *reinterpret_cast<const bf16x8*>(ga + kn);
+            rb = *reinterpret_cast<const bf16x8*>(gb + kn);
         }
-        bf16x8 a[4], b[4];
+        bf16x8 a[4], b[2];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][64 * wi + 16 * i + r][kq]);
-            b[i] = *reinterpret_cast<const bf16x8*>(&Bs[cur][64 * wj + 16 * i + r][kq]);
-        }
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][64 * wi + 16 * i + r][kq]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][32 * wj + 16 * j + r][kq]);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
-        if (v % ns == ns - 1) {  // slice v / ns complete
+            for (int j = 0; j < 2; ++j) p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
+        if (v % ns == ns - 1) {
             const bool first = v < ns;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < 2; ++j) {
                     sum[i][j] = first ? p[i][j] : sum[i][j] + p[i][j];
                     p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                 }
         }
         if (more) {
-            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr][lc]) = ra0;
-            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr + 64][lc]) = ra1;
-            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr][lc]) = rb0;
-            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr + 64][lc]) = rb1;
+            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr][lc]) = ra;
+            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr][lc]) = rb;
         }
         __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = n0 + 64 * wj + 16 * j + r;
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + 32 * wj + 16 * j + r;
             const int mb = m0 + 64 * wi + 16 * i + ((lane >> 4) << 2);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -704,7 +692,7 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
     // k_gemm128 and k_gemm64)
     if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 128 == 0 && g.N % 128 == 0 &&
         g.K % 128 == 0) {
-        const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(256);
+        const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(512);
         if (epi == EPI_HIDDEN)
             k_gemm128<EPI_HIDDEN><<<grid128, block128, 0, s->stream>>>(g);
         else
