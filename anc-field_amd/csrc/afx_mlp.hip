@@ -17,8 +17,9 @@
 //                      OUT     y = C + b (FP32)
 //                      LOSS    y = C + b; g = 2 w (y - t) -> BF16 both layouts; per-wave loss partial
 //                      BACK    g = C * silu'(z_prev) -> BF16 both layouts
-//   k_gemm128<EPI>   HIDDEN and OUT for inference chunks: 128x128 tiles through
-//                    LDS, k-slices visited in turn (same bits as k_gemm)
+//   k_gemm128<EPI>   HIDDEN and OUT for inference chunks (K a multiple of 256):
+//                    128x128 tiles through LDS, k-slices visited in turn (same
+//                    bits as k_gemm)
 //   k_gemm64<EPI>    HIDDEN and OUT for inference chunks: 64x64 tiles through LDS,
 //                    the same bits as k_gemm
 //   k_grads          all weight gradients of a step, dW_l = dY_l^T . X_l (A = dY_l^T
@@ -354,33 +355,39 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
 
 // Inference GEMM for large M on 128x128 output tiles (HIDDEN and OUT
 // epilogues): eight waves, each a 64x32 part of 4x2 MFMA blocks, operands
-// staged through LDS in k-steps of 32 (double buffered, one barrier per
-// step); 101 VGPRs, so four waves per SIMD hide the operand loads.  The bits are k_gemm's: every MFMA sees the same operands per lane,
+// staged through LDS two k-steps of 32 at a time (double buffered, 80 KiB,
+// one barrier per pair); 110 VGPRs, so four waves per SIMD hide the operand
+// loads.  The bits are k_gemm's: every MFMA sees the same operands per lane,
 // and each output is the sum of four k-slice partials (slice = k-step mod 4)
 // in k_gemm's order.  The k-steps are visited slice by slice (0, 4, 8, ...,
 // then 1, 5, 9, ...), so only the current slice's partials and the running
 // sum of the finished slices are held: ((p0 + p1) + p2) + p3 as in k_gemm64,
-// with half its accumulator registers per output.  Needs M, N and K
-// multiples of 128.
+// with half its accumulator registers per output.  Needs M and N multiples
+// of 128 and K of 256 (a pair of k-steps never straddles two slices).
 template <int E>
 __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
 {
     constexpr int kPad = 40;  // LDS row stride in bf16 (32 + 8)
-    __shared__ bf16 As[2][128][kPad];
-    __shared__ bf16 Bs[2][128][kPad];
+    __shared__ bf16 As[2][2][128][kPad];  // [buffer][k-step of the pair][row][k]
+    __shared__ bf16 Bs[2][2][128][kPad];
     const int tiles_n = g.N >> 7;
     const int m0 = (int)(blockIdx.x / tiles_n) << 7;
     const int n0 = (int)(blockIdx.x % tiles_n) << 7;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = wave >> 2, wj = wave & 3;  // 64-row half, 32-column quarter
-    const int lr = tid >> 2, lc = (tid & 3) << 3;  // row lr, one 16-byte piece of each k-step
+    const int wi = wave >> 2, wj = wave & 3;
+    const int lr = tid >> 2, lc = (tid & 3) << 3;
     const bf16* ga = g.A + (size_t)(m0 + lr) * g.lda + lc;
     const bf16* gb = g.B + (size_t)(n0 + lr) * g.ldb + lc;
-    const int nk = g.K >> 5, ns = nk >> 2;
-    bf16x8 ra = *reinterpret_cast<const bf16x8*>(ga);
-    bf16x8 rb = *reinterpret_cast<const bf16x8*>(gb);
-    *reinterpret_cast<bf16x8*>(&As[0][lr][lc]) = ra;
-    *reinterpret_cast<bf16x8*>(&Bs[0][lr][lc]) = rb;
+    const int nk = g.K >> 5, ns = nk >> 2;  // ns even: a pair never straddles two slices
+    auto kof = [&](int v) { return 32 * (v / ns + 4 * (v % ns)); };
+    bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(ga + kof(0));
+    bf16x8 ra1 = *reinterpret_cast<const bf16x8*>(ga + kof(1));
+    bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(gb + kof(0));
+    bf16x8 rb1 = *reinterpret_cast<const bf16x8*>(gb + kof(1));
+    *reinterpret_cast<bf16x8*>(&As[0][0][lr][lc]) = ra0;
+    *reinterpret_cast<bf16x8*>(&As[0][1][lr][lc]) = ra1;
+    *reinterpret_cast<bf16x8*>(&Bs[0][0][lr][lc]) = rb0;
+    *reinterpret_cast<bf16x8*>(&Bs[0][1][lr][lc]) = rb1;
     __syncthreads();
     const int r = lane & 15, kq = (lane >> 4) << 3;
     f32x4 sum[4][2], p[4][2];
@@ -391,25 +398,30 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
             sum[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
-    for (int v = 0; v < nk; ++v) {
-        const int cur = v & 1;
-        const bool more = v + 1 < nk;
+    for (int v = 0; v < nk; v += 2) {
+        const int cur = (v >> 1) & 1;
+        const bool more = v + 2 < nk;
         if (more) {
-            const int kn = 32 * ((v + 1) / ns + 4 * ((v + 1) % ns));
-            ra = *reinterpret_cast<const bf16x8*>(ga + kn);
-            rb = *reinterpret_cast<const bf16x8*>(gb + kn);
+            ra0 = *reinterpret_cast<const bf16x8*>(ga + kof(v + 2));
+            ra1 = *reinterpret_cast<const bf16x8*>(ga + kof(v + 3));
+            rb0 = *reinterpret_cast<const bf16x8*>(gb + kof(v + 2));
+            rb1 = *reinterpret_cast<const bf16x8*>(gb + kof(v + 3));
         }
-        bf16x8 a[4], b[2];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][64 * wi + 16 * i + r][kq]);
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 a[4], b[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][32 * wj + 16 * j + r][kq]);
+            for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][h][64 * wi + 16 * i + r][kq]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][h][32 * wj + 16 * j + r][kq]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
-        if (v % ns == ns - 1) {
-            const bool first = v < ns;
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
+        }
+        if ((v + 1) % ns == ns - 1) {  // slice (v + 1) / ns complete
+            const bool first = v + 1 < ns;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -419,8 +431,10 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
                 }
         }
         if (more) {
-            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr][lc]) = ra;
-            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr][lc]) = rb;
+            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][0][lr][lc]) = ra0;
+            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][1][lr][lc]) = ra1;
+            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][0][lr][lc]) = rb0;
+            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][1][lr][lc]) = rb1;
         }
         __syncthreads();
     }
@@ -691,7 +705,7 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
     // large-M inference GEMMs on 128x128 or 64x64 tiles (same bits; see
     // k_gemm128 and k_gemm64)
     if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 128 == 0 && g.N % 128 == 0 &&
-        g.K % 128 == 0) {
+        g.K % 256 == 0) {
         const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(512);
         if (epi == EPI_HIDDEN)
             k_gemm128<EPI_HIDDEN><<<grid128, block128, 0, s->stream>>>(g);
