@@ -75,6 +75,62 @@ def test_repl_ie224_run_and_save(tmp_path):
 
 
 @pytest.mark.gpu
+def test_repl_device_and_setup_commands(tmp_path):
+    """lsd, sd, cd, sdt and vis answer as the command table says
+    (commandline.hpp:107-196); minsize and pml reach the simulation: a
+    session with both gives the library's traces for the same calls."""
+    print("\n[case] REPL device commands, minsize and pml", flush=True)
+    afx = pytest.importorskip("anc_field_py._afx")
+    model = os.path.join(GOLD, "ie224", "model.xml")
+    out = repl(["lsd", "sd [0]", "cd", "sdt cpu", "sdt tpu", "vis 0.5"])
+    parts = [l.strip() for l in out.split(">> ") if l.strip()]
+    assert parts[0].startswith("Available compute devices") and "\n0. " in parts[0]
+    assert parts[1].endswith("added to active devices")
+    assert parts[2] == "Compute devices removed"
+    assert parts[3].startswith("Error:")
+    assert parts[4] == "Valid command arguments are 'cpu' or 'gpu'"
+    assert parts[5].startswith("Error:")
+    s = afx.Sim()
+    s.load_kerkythea(model)
+    s.run(0)
+    st = s.stats()
+    nx, ny, nz = st.nx, st.ny, st.nz
+    s.close()
+    size = [float(np.float32(f"{v:.4f}")) for v in ((nx + 32) * 0.02, ny * 0.02, nz * 0.02)]
+    seconds = 0.01
+    steps = int(np.float32(seconds) * np.float32(32000))
+    src, sig = _src_file(tmp_path, steps + 10)
+    cmds = ["sd [0]", f"load {model}", "minsize [%.4f,%.4f,%.4f]" % tuple(size), "pml 8",
+            f"addsource [{','.join(map(str, SOURCES['actuator']))}] {src}"]
+    cmds += [f"addmic [{','.join(map(str, m))}]" for m in MICS]
+    cmds += [f"run {seconds}", f"save {tmp_path / 'out.dat'}"]
+    out = repl(cmds)
+    assert f"{steps}/{steps}" in out and "Error" not in out
+    saved = np.loadtxt(tmp_path / "out.dat")
+    runs = []
+    for setup in (True, False):
+        s = afx.Sim()
+        s.load_kerkythea(model)
+        if setup:
+            s.set_min_size(*size)
+            s.set_pml(8)
+        s.add_source(SOURCES["actuator"], np.loadtxt(src).astype(np.float32))
+        for m in MICS:
+            s.add_receiver(m)
+        s.run(steps)
+        st = s.stats()
+        runs.append((s.receivers(), (st.nx, st.ny, st.nz)))
+        s.close()
+    (lib, dims_set), (plain, dims_plain) = runs
+    # SetMinSize: RoundTo((size_t)(x / res) + 1, 32) in single precision (simulation.hpp:279-284)
+    want = tuple(int(-(-(int(np.float32(v) / np.float32(0.02)) + 1) // 32) * 32) for v in size)
+    assert dims_set == want and dims_plain == (nx, ny, nz), (dims_set, want)
+    assert want[0] > nx
+    assert np.allclose(saved, lib, rtol=1e-5, atol=1e-12)
+    assert not np.allclose(lib, plain, rtol=1e-3, atol=1e-9)
+
+
+@pytest.mark.gpu
 def test_ancfield_matches_library():
     print("\n[case] AncField ie224 0.02 s", flush=True)
     from anc_field_py import _afx, ancutil
