@@ -492,17 +492,15 @@ __device__ inline void grads_tile(const GradTable& t, AdamArgs ad, const StepRef
     *reinterpret_cast<bf16x4*>(g.wt + (size_t)n * g.M + mb) = col;
 }
 
-// Fourier features: thread (b, j, k) writes sin and cos of 2 pi f_k x[b][j]
-// at features j*2F + k and j*2F + F + k, in X [rows][nfeat] and X^T [nfeat][ldt].
+// Fourier features: a block holds blockDim.z rows; thread (k, j, r) writes
+// sin and cos of 2 pi f_k x[b][j], b = blockIdx.x * blockDim.z + r, at
+// features j*2F + k and j*2F + F + k, in X [rows][nfeat] and X^T [nfeat][ldt].
 __global__ void k_features(const float* __restrict__ x, const int* __restrict__ rows, int nrows, int in_dim, int nf,
                            const float* __restrict__ freq, bf16* __restrict__ X, bf16* __restrict__ XT, int ldt,
                            StepRef sr)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrows * in_dim * nf) return;
-    const int k = i % nf;
-    const int j = (i / nf) % in_dim;
-    const int b = i / (nf * in_dim);
+    const int k = (int)threadIdx.x, j = (int)threadIdx.y, b = (int)(blockIdx.x * blockDim.z + threadIdx.z);
+    if (b >= nrows || k >= nf || j >= in_dim) return;
     const int src = rows ? rows[step_of(sr) * sr.rows_stride + b] : b;
     const float cyc = freq[k] * x[(size_t)src * in_dim + j];
     float s, c;
@@ -738,12 +736,22 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
 // Features of `nrows` positions (gathered through rows when not null), then
 // the hidden layers.  With keep set, pre-activations and the column-major
 // copies needed by the backward pass are written.
+hipError_t launch_features(afx_mlp* s, const float* x, const int* rows, int nrows, bf16* X, bf16* XT, int ldt,
+                           StepRef sr)
+{
+    // nf x in_dim threads per row (at most 1024, checked at creation), rows
+    // stacked in z up to about 256 threads per block
+    const int per_row = s->cfg.n_freq * s->cfg.in_dim;
+    const int rz = std::max(1, std::min(256 / per_row, 1024 / per_row));
+    const dim3 block((unsigned)s->cfg.n_freq, (unsigned)s->cfg.in_dim, (unsigned)rz);
+    k_features<<<dim3((unsigned)((nrows + rz - 1) / rz)), block, 0, s->stream>>>(x, rows, nrows, s->cfg.in_dim,
+                                                                               s->cfg.n_freq, s->freq, X, XT, ldt, sr);
+    return hipGetLastError();
+}
+
 int forward_hidden(afx_mlp* s, const float* x, const int* rows, bool keep)
 {
-    const int nthreads = s->B * s->cfg.in_dim * s->cfg.n_freq;
-    k_features<<<(nthreads + 255) / 256, 256, 0, s->stream>>>(x, rows, s->B, s->cfg.in_dim, s->cfg.n_freq, s->freq,
-                                                             s->X, keep ? s->XT : nullptr, s->B, s->sr);
-    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(launch_features(s, x, rows, s->B, s->X, keep ? s->XT : nullptr, s->B, s->sr));
     for (int l = 0; l + 1 < s->L; ++l) {
         Gemm g{};
         g.A = l == 0 ? s->X : s->A[l - 1];
@@ -939,11 +947,8 @@ int ensure_infer(afx_mlp* s)
 // change a value.
 int forward_chunk(afx_mlp* s, int m, float* y)
 {
-    const int I = s->cfg.in_dim, nf = s->cfg.n_freq, L = s->L;
-    const int nthreads = m * I * nf;
-    k_features<<<(nthreads + 255) / 256, 256, 0, s->stream>>>(s->xqi, nullptr, m, I, nf, s->freq, s->Xi, nullptr, m,
-                                                              StepRef{});
-    HIP_CHECK(hipGetLastError());
+    const int L = s->L;
+    HIP_CHECK(launch_features(s, s->xqi, nullptr, m, s->Xi, nullptr, m, StepRef{}));
     const bf16* act = s->Xi;
     for (int l = 0; l < L; ++l) {
         Gemm g{};
@@ -1033,6 +1038,8 @@ int afx_mlp_create(const afx_mlp_config* cfg, afx_mlp** out)
     const afx_mlp_config& c = *cfg;
     if (c.in_dim < 1 || c.n_freq < 1 || c.depth < 2 || c.depth > kMaxLayers)
         return err(AFX_E_ARG, "need in_dim >= 1, n_freq >= 1 and 2 <= depth <= %d", kMaxLayers);
+    if (c.in_dim * c.n_freq > 1024)
+        return err(AFX_E_ARG, "in_dim * n_freq = %d above 1024 (one feature workgroup per row)", c.in_dim * c.n_freq);
     const int F = 2 * c.in_dim * c.n_freq;
     if (F % 32 || c.width % 32 || c.out_dim % 32 || c.batch % 32 || c.width < 32 || c.out_dim < 32 || c.batch < 32)
         return err(AFX_E_ARG, "features (%d), width, out_dim and batch must be positive multiples of 32", F);

@@ -12,6 +12,7 @@ mode, run through the C ABI.  Prints one JSON line; its extra legs are the
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
                   [--compat] [--no-cpu-baseline] [--no-tune] [--no-batched] [--no-surrogate]
                   [--no-zone] [--no-large] [--large-precisions LIST] [--copy-probe BYTES]
+                  [--mlp-inference ROWS]
 """
 import argparse
 import json
@@ -337,6 +338,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=4096)
+    ap.add_argument("--mlp-inference", type=int, default=0, metavar="ROWS",
+                    help="only time the surrogate network's forward pass on ROWS device-resident queries")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and print its rate")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -363,6 +366,21 @@ def main():
     if args.copy_probe:
         print(json.dumps({"copy_bytes": args.copy_probe, "reps": 3,
                           "gbps": round(_afx.copy_probe(args.copy_probe, reps=3), 1)}), flush=True)
+        return
+    if args.mlp_inference:
+        # the 6x512 network (random weights) on uniform queries in HBM, best of six
+        import numpy as np
+        rows = args.mlp_inference
+        xq = np.random.default_rng(0).uniform(0, 1, (rows, 6)).astype(np.float32)
+        with _afx.Mlp(seed=3) as m, _afx.DeviceBuffer(xq.nbytes) as xd, \
+                _afx.DeviceBuffer(rows * m.cfg.out_dim * 4) as yd:
+            xd.upload(xq)
+            best = float("inf")
+            for _ in range(6):
+                m.forward_device(xd, rows, yd)
+                best = min(best, m.stats().last_forward_gpu_ms)
+        print(json.dumps({"mlp_inference_rows": rows, "gpu_ms": round(best, 4),
+                          "impulse_responses_per_s": round(rows / (best * 1e-3), 1)}), flush=True)
         return
 
     total = args.warmup + args.steps + args.profile_steps

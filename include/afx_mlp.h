@@ -39,7 +39,8 @@ typedef struct afx_mlp afx_mlp;
 
 typedef struct afx_mlp_config {
     int32_t in_dim;       /* coordinates per query (6: source xyz, receiver xyz) */
-    int32_t n_freq;       /* Fourier frequencies per coordinate; 2*in_dim*n_freq must be a multiple of 32 */
+    int32_t n_freq;       /* Fourier frequencies per coordinate; 2*in_dim*n_freq must be a multiple of 32,
+                             in_dim*n_freq at most 1024 */
     int32_t width;        /* hidden units per layer, multiple of 32 */
     int32_t depth;        /* linear layers, >= 2 */
     int32_t out_dim;      /* outputs per query, multiple of 32 */
