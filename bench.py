@@ -12,7 +12,7 @@ mode, run through the C ABI.  Prints one JSON line; its extra legs are the
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
                   [--compat] [--no-cpu-baseline] [--no-tune] [--no-batched] [--no-surrogate]
                   [--no-zone] [--no-large] [--large-precisions LIST] [--copy-probe BYTES]
-                  [--mlp-inference ROWS]
+                  [--mlp-train STEPS] [--mlp-inference ROWS]
 """
 import argparse
 import json
@@ -338,6 +338,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=4096)
+    ap.add_argument("--mlp-train", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS surrogate training steps at batch 512 on synthetic data")
     ap.add_argument("--mlp-inference", type=int, default=0, metavar="ROWS",
                     help="only time the surrogate network's forward pass on ROWS device-resident queries")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -366,6 +368,24 @@ def main():
     if args.copy_probe:
         print(json.dumps({"copy_bytes": args.copy_probe, "reps": 3,
                           "gbps": round(_afx.copy_probe(args.copy_probe, reps=3), 1)}), flush=True)
+        return
+    if args.mlp_train:
+        # the 6x512 network at batch 512 on 4096 synthetic pairs; with
+        # --surrogate-eager the steps are launched one by one instead of
+        # replayed from captured graphs
+        import numpy as np
+        rng = np.random.default_rng(0)
+        n = 4096
+        with _afx.Mlp(seed=3, batch=512, flags=1 if args.surrogate_eager else 0) as m:
+            xs = rng.uniform(0, 1, (n, 6)).astype(np.float32)
+            ts = (0.05 * rng.standard_normal((n, m.cfg.out_dim))).astype(np.float32)
+            m.set_data(xs, ts, np.full(n, 1.0 / n, np.float32))
+            m.train(64, 2e-3, 1e-5)  # warm-up, graph capture
+            m.train(args.mlp_train, 2e-3, 1e-5)
+            st = m.stats()
+        per = st.last_train_gpu_ms / args.mlp_train
+        print(json.dumps({"mlp_train_steps": args.mlp_train, "batch": 512, "gpu_ms_per_step": round(per, 4),
+                          "tflops": round(st.flop_per_step / (per * 1e-3) / 1e12, 1)}), flush=True)
         return
     if args.mlp_inference:
         # the 6x512 network (random weights) on uniform queries in HBM, best of six
