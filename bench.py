@@ -12,7 +12,7 @@ mode, run through the C ABI.  Prints one JSON line; its extra legs are the
   python bench.py [--steps K] [--warmup W] [--precision fp32|fp16|bf16|fp64]
                   [--compat] [--no-cpu-baseline] [--no-tune] [--no-batched] [--no-surrogate]
                   [--no-zone] [--no-large] [--large-precisions LIST] [--copy-probe BYTES]
-                  [--mlp-train STEPS] [--mlp-inference ROWS]
+                  [--ie224 SECONDS] [--mlp-train STEPS] [--mlp-inference ROWS]
 """
 import argparse
 import json
@@ -338,6 +338,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=4096)
+    ap.add_argument("--ie224", type=float, default=0.0, metavar="SECONDS",
+                    help="only time the ie224 field simulation for SECONDS of audio (32 kHz steps)")
     ap.add_argument("--mlp-train", type=int, default=0, metavar="STEPS",
                     help="only time STEPS surrogate training steps at batch 512 on synthetic data")
     ap.add_argument("--mlp-inference", type=int, default=0, metavar="ROWS",
@@ -368,6 +370,42 @@ def main():
     if args.copy_probe:
         print(json.dumps({"copy_bytes": args.copy_probe, "reps": 3,
                           "gbps": round(_afx.copy_probe(args.copy_probe, reps=3), 1)}), flush=True)
+        return
+    if args.ie224:
+        # The reference's own timed run (examples/ie224-noise-results/report.md:3,9):
+        # the ie224 room (416x160x576 cells from its Kerkythea model, here the
+        # test fixture copy), white noise at the loudspeaker, 7 receivers, the
+        # reference's constants and compat mode, FP32; it reports about 10
+        # minutes for 10 s of audio on a GTX 1070.
+        import numpy as np
+        here = os.path.dirname(os.path.abspath(__file__))
+        model = os.path.join(here, "tests", "golden", "ie224", "model.xml")
+        steps = int(np.float32(args.ie224) * np.float32(32000))  # main.cpp:117-121
+        sig = np.random.default_rng(0).standard_normal(steps).astype(np.float32)
+        mics = [[4, 1.6, 5.3], [4, 1.6, 4.5], [4, 1.6, 6.5], [3, 1.6, 5.5], [5, 1.6, 5.5], [4, 0.6, 5.5],
+                [4, 2.6, 5.5]]
+        s = _afx.Sim()
+        s.load_kerkythea(model)
+        s.add_source([4, 1.6, 1.0], sig)
+        for m in mics:
+            s.add_receiver(m)
+        s.run(0)
+        st = s.stats()
+        cells = st.nx * st.ny * st.nz
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:  # progress every 32 000 steps
+            k = min(32000, steps - done)
+            s.run(k)
+            s.synchronize()
+            done += k
+            progress(f"ie224: {done}/{steps} steps")
+        wall = time.perf_counter() - t0
+        tr = s.receivers()
+        s.close()
+        print(json.dumps({"ie224_audio_s": args.ie224, "steps": steps, "cells": cells, "wall_s": round(wall, 2),
+                          "gcell_per_s": round(cells * steps / wall / 1e9, 1), "finite": bool(np.all(np.isfinite(tr))),
+                          "reference": "about 10 minutes for 10 s of audio on a GTX 1070"}), flush=True)
         return
     if args.mlp_train:
         # the 6x512 network at batch 512 on 4096 synthetic pairs; with
