@@ -392,6 +392,10 @@ def main():
         s.run(0)
         st = s.stats()
         cells = st.nx * st.ny * st.nz
+        tuned = None
+        if not args.no_tune:  # tile shape and steps per pass; the state is kept
+            tuned = list(s.autotune(30)) + [s.time_block()[0]]
+        s.synchronize()
         t0 = time.perf_counter()
         done = 0
         while done < steps:  # progress every 32 000 steps
@@ -405,6 +409,7 @@ def main():
         s.close()
         print(json.dumps({"ie224_audio_s": args.ie224, "steps": steps, "cells": cells, "wall_s": round(wall, 2),
                           "gcell_per_s": round(cells * steps / wall / 1e9, 1), "finite": bool(np.all(np.isfinite(tr))),
+                          "tuned_tile_and_steps_per_pass": tuned,
                           "reference": "about 10 minutes for 10 s of audio on a GTX 1070"}), flush=True)
         return
     if args.mlp_train:
