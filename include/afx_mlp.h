@@ -79,7 +79,7 @@ int afx_mlp_set_params(afx_mlp* mlp, const float* in, int64_t n);
 int afx_mlp_get_frequencies(afx_mlp* mlp, float* out, int32_t cap);
 
 /* y[rows][out_dim] for x[rows][in_dim]; any rows >= 1.  Rows are processed
- * in chunks of up to 8192 with inference buffers of their own (allocated on
+ * in chunks of up to 32 768 with inference buffers of their own (allocated on
  * the first call); each output row depends on its input row only, so the
  * values do not depend on the chunking or on the batch size. */
 int afx_mlp_forward(afx_mlp* mlp, const float* x, int64_t rows, float* y);
