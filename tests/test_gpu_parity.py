@@ -222,19 +222,23 @@ def test_batched_rooms_match_single():
         k += len(w.mic_cells)
 
 
-@pytest.mark.parametrize("precision,tblock", [("fp32", 1), ("fp32", 2), ("fp16", 1)])
-def test_batched_rooms_match_oracle(precision, tblock):
-    """Each room of one handle, all with a physical PML on x+ and z-, against
-    its own oracle run, bit for bit."""
-    _case(f"batched rooms vs the oracle per room, {precision}, {tblock} step(s) per pass")
+@pytest.mark.parametrize("precision,tblock,compat", [("fp32", 1, False), ("fp32", 2, False), ("fp16", 1, False),
+                                                    ("fp32", 2, True), ("fp16", 1, True)])
+def test_batched_rooms_match_oracle(precision, tblock, compat):
+    """Each room of one handle against its own oracle run, bit for bit:
+    physical mode with a PML on x+ and z- of every room, or compat mode (the
+    reference's constants; its PML is single-room only, so none)."""
+    _case(f"batched rooms vs the oracle per room, {precision}, {tblock} step(s) per pass, compat {compat}")
     ws = rooms.batch_rooms(seed=11, nrooms=3, steps=130, lo=(70, 24, 24), hi=(140, 40, 40), nmic=3)
     faces = afx.FACE_XP | afx.FACE_ZM
     width = 5
     prof = (3e4 * ((np.arange(width) + 0.5) / width) ** 3).astype(np.float32)
-    s = afx.Sim(c=ws[0].c, dx=ws[0].dx, dt=ws[0].dt, res=ws[0].dx, compat=False, precision=precision)
+    c, dt = (340.0, 3.125e-05) if compat else (ws[0].c, ws[0].dt)
+    s = afx.Sim(c=c, dx=ws[0].dx, dt=dt, res=ws[0].dx, compat=compat, precision=precision)
     s.set_voxels(ws[0].vox)
     s.set_materials(ws[0].a, ws[0].b)
-    s.set_pml(width, faces, prof)
+    if not compat:
+        s.set_pml(width, faces, prof)
     s.set_time_block(tblock)
     ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
     for rid, w in zip(ids, ws):
@@ -248,8 +252,9 @@ def test_batched_rooms_match_oracle(precision, tblock):
     s.close()
     k = 0
     for w in ws:
-        o = orc.Oracle(w.vox, w.a, w.b, w.c, w.dx, w.dt, False, w.src_cells, w.src, w.mic_cells, "f32", 2, width,
-                       faces, prof, storage=precision if precision != "fp32" else None)
+        o = orc.Oracle(w.vox, w.a, w.b, c, w.dx, dt, compat, w.src_cells, w.src, w.mic_cells, "f32",
+                       0 if compat else 2, 0 if compat else width, 0 if compat else faces, None if compat else prof,
+                       storage=precision if precision != "fp32" else None)
         want = o.run(w.steps)
         _assert_bitexact(tr[k:k + len(w.mic_cells)], want, w.name)
         assert np.abs(want).max() > 0
