@@ -53,3 +53,25 @@ def test_oracle_reproduces_reference_actuator_traces():
         rel = np.linalg.norm(got[i] - want[i]) / np.linalg.norm(want[i])
         # the fixture carries 6 significant digits: 1e-6 relative is its resolution
         assert rel < 5e-6, (i, rel)
+
+
+def test_dsp_model_voxelizes_like_the_reference():
+    """The reference's dsp room (tests/golden/dsp): the grid its loader
+    builds, materials numbered in sorted-name order with the coefficients
+    read from <name>_a.dat / _b.dat (kerkythea.hpp:63-196, util.hpp:101-112),
+    and the same boundary groups from the library's classifier and the
+    oracle's."""
+    d = os.path.join(GOLD, "dsp")
+    vox, a, b = _afx.voxelize_kerkythea(os.path.join(d, "model.xml"))
+    assert vox.shape == (384, 128, 256)
+    names = sorted(f[:-6] for f in os.listdir(d) if f.endswith("_a.dat"))
+    assert names == ["absorber", "concrete", "door", "glass", "wood"]
+    assert a.shape == b.shape == (5, 4)
+    for k, n in enumerate(names):
+        assert np.array_equal(a[k], np.loadtxt(os.path.join(d, f"{n}_a.dat"), dtype=np.float32))
+        assert np.array_equal(b[k], np.loadtxt(os.path.join(d, f"{n}_b.dat"), dtype=np.float32))
+    assert set(np.unique(vox).tolist()) <= set(range(-1, 5)) and (vox == -1).any()
+    counts = _afx.classify_counts(vox, compat=True)
+    _, grp = orc.classify(vox)
+    assert counts.sum() > 0
+    assert np.array_equal(counts, np.bincount(grp, minlength=26))
