@@ -56,3 +56,34 @@ def test_dsp_model_matches_oracle(tblock):
     assert np.array_equal(got, o.receivers().astype(got.dtype)), "traces differ"
     assert np.array_equal(f, o.field(0).astype(f.dtype)), "fields differ"
     assert np.abs(got).max() > 0
+
+
+def test_ie224_fp64_matches_oracle():
+    """The ie224 room (416x160x576) in FP64, compat mode, the reference's
+    actuator source and 7 receivers: 200 steps against the oracle's double
+    build, bit for bit (traces come back as FP32 copies of FP64 state)."""
+    print("\n[case] ie224 model, FP64, compat, 200 steps", flush=True)
+    model = os.path.join(os.path.dirname(MODEL), "..", "ie224", "model.xml")
+    v, a, b = afx.voxelize_kerkythea(model)
+    steps = 200
+    dt = 3.125e-05
+    sig = rooms.ricker(steps, dt, 800.0)
+    src_pos = [4, 1.6, 5.5]
+    mic_pos = [[4, 1.6, 5.3], [4, 1.6, 4.5], [4, 1.6, 6.5], [3, 1.6, 5.5], [5, 1.6, 5.5], [4, 0.6, 5.5], [4, 2.6, 5.5]]
+    s = afx.Sim(precision="fp64")
+    s.load_kerkythea(model)
+    s.add_source(src_pos, sig)
+    for m in mic_pos:
+        s.add_receiver(m)
+    s.run(steps)
+    got = s.receivers()
+    s.close()
+
+    def cell(p):  # (cl_int)(pos / res_) in single precision, simulation.hpp:322,338
+        return [int(np.float32(q) / np.float32(0.02)) for q in p]
+
+    o = orc.Oracle(v, a, b, 340.0, 0.02, dt, True, [cell(src_pos)], sig[None], [cell(m) for m in mic_pos], "f64",
+                   threads=16)
+    want = o.run(steps)
+    assert np.array_equal(got, want.astype(np.float32)), "traces differ"
+    assert np.abs(got).max() > 0
