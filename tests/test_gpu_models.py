@@ -58,11 +58,13 @@ def test_dsp_model_matches_oracle(tblock):
     assert np.abs(got).max() > 0
 
 
-def test_ie224_fp64_matches_oracle():
-    """The ie224 room (416x160x576) in FP64, compat mode, the reference's
-    actuator source and 7 receivers: 200 steps against the oracle's double
-    build, bit for bit (traces come back as FP32 copies of FP64 state)."""
-    print("\n[case] ie224 model, FP64, compat, 200 steps", flush=True)
+@pytest.mark.parametrize("precision", ["fp64", "fp16", "bf16"])
+def test_ie224_precisions_match_oracle(precision):
+    """The ie224 room (416x160x576), compat mode, the reference's actuator
+    source and 7 receivers, 200 steps against the oracle bit for bit: FP64
+    against its double build (traces come back as FP32 copies of FP64
+    state), FP16/BF16 storage against its storage mode."""
+    print(f"\n[case] ie224 model, {precision}, compat, 200 steps", flush=True)
     model = os.path.join(os.path.dirname(MODEL), "..", "ie224", "model.xml")
     v, a, b = afx.voxelize_kerkythea(model)
     steps = 200
@@ -70,7 +72,7 @@ def test_ie224_fp64_matches_oracle():
     sig = rooms.ricker(steps, dt, 800.0)
     src_pos = [4, 1.6, 5.5]
     mic_pos = [[4, 1.6, 5.3], [4, 1.6, 4.5], [4, 1.6, 6.5], [3, 1.6, 5.5], [5, 1.6, 5.5], [4, 0.6, 5.5], [4, 2.6, 5.5]]
-    s = afx.Sim(precision="fp64")
+    s = afx.Sim(precision=precision)
     s.load_kerkythea(model)
     s.add_source(src_pos, sig)
     for m in mic_pos:
@@ -82,8 +84,9 @@ def test_ie224_fp64_matches_oracle():
     def cell(p):  # (cl_int)(pos / res_) in single precision, simulation.hpp:322,338
         return [int(np.float32(q) / np.float32(0.02)) for q in p]
 
-    o = orc.Oracle(v, a, b, 340.0, 0.02, dt, True, [cell(src_pos)], sig[None], [cell(m) for m in mic_pos], "f64",
-                   threads=16)
+    o = orc.Oracle(v, a, b, 340.0, 0.02, dt, True, [cell(src_pos)], sig[None], [cell(m) for m in mic_pos],
+                   "f64" if precision == "fp64" else "f32", threads=16,
+                   storage=None if precision == "fp64" else precision)
     want = o.run(steps)
     assert np.array_equal(got, want.astype(np.float32)), "traces differ"
     assert np.abs(got).max() > 0
