@@ -168,6 +168,10 @@ struct afx_sim {
     std::vector<Mic> mics;
     int tile = 0;
     int tblock = 1;  // steps per pass over the interior tiles (1, or 2: work kind 4)
+    // the XCD of a tile's y-band advances by one per z-layer of tiles (spreads
+    // y-face walls and PML slabs over the XCDs); off, a band stays on one XCD
+    // through z, so z-neighbour tiles share its L2.  The autotuner chooses.
+    bool xcd_rotate = true;
     bool fused_ok = true;
 
     // ---- device state
@@ -302,10 +306,12 @@ hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t st)
 // Tile table in XCD-swizzled order.  Workgroup i runs on XCD i % 8.
 // Logical tiles (x fastest, then y, then z) are dealt to the XCDs in groups
 // of G consecutive tiles, G an eighth of room 0's z-layer of tiles, round
-// robin with the starting XCD advanced by one per layer: every XCD gets a
-// share of each layer (the z walls) and of each band of y (the y walls and
-// slabs), and each group is a run of y-adjacent tiles whose halos stay in
-// that XCD's L2.  Slot i holds the (i / 8)-th tile of XCD i % 8.
+// robin with the starting XCD advanced by one per layer (xcd_rotate): every
+// XCD gets a share of each layer (the z walls) and of each band of y (the y
+// walls and slabs), and each group is a run of y-adjacent tiles whose halos
+// stay in that XCD's L2.  Without the rotation a band keeps its XCD through
+// z, so z-halo planes stay in L2 too, but y-face work sits on two XCDs.
+// Slot i holds the (i / 8)-th tile of XCD i % 8.
 void build_tiles(afx_sim* s, std::vector<int4>& table, std::vector<int32_t>& logical_of_slot)
 {
     const TileShape ts = kTiles[s->tile];
@@ -321,7 +327,9 @@ void build_tiles(afx_sim* s, std::vector<int4>& table, std::vector<int32_t>& log
     const int64_t layer = (int64_t)((r0.nx + ts.tx - 1) / ts.tx) * ((r0.ny + ts.ty - 1) / ts.ty);
     const int64_t G = std::max<int64_t>(1, layer / 8);
     std::vector<std::vector<int64_t>> of_xcd(8);
-    for (int64_t L = 0; L < n; ++L) of_xcd[(size_t)((L / G + L / std::max<int64_t>(layer, 1)) % 8)].push_back(L);
+    const bool rot = s->xcd_rotate;
+    for (int64_t L = 0; L < n; ++L)
+        of_xcd[(size_t)((L / G + (rot ? L / std::max<int64_t>(layer, 1) : 0)) % 8)].push_back(L);
     int64_t per = 0;
     for (const auto& v : of_xcd) per = std::max<int64_t>(per, (int64_t)v.size());
     table.assign((size_t)(per * 8), make_int4(-1, 0, 0, 0));
@@ -2062,12 +2070,14 @@ int snap_restore(afx_sim* s, const Snapshot& sn)
     return AFX_OK;
 }
 
-// Tile shape and steps per pass of a committed handle: the fourth field is
-// allocated (zeroed) the first time two steps per pass are chosen.
-int set_tile_tb(afx_sim* s, int tile, int tb)
+// Tile shape, steps per pass and XCD layer rotation of a committed handle:
+// the fourth field is allocated (zeroed) the first time two steps per pass
+// are chosen.
+int set_tile_tb(afx_sim* s, int tile, int tb, bool rotate)
 {
     s->tile = tile;
     s->tblock = tb;
+    s->xcd_rotate = rotate;
     s->drop_graphs();
     if (tb == 2 && !s->P[3].p) {
         HIP_TRY(s->P[3].alloc(s->P[0].bytes));
@@ -2078,7 +2088,7 @@ int set_tile_tb(afx_sim* s, int tile, int tb)
     return build_special(s);
 }
 
-int set_tile(afx_sim* s, int tile) { return set_tile_tb(s, tile, s->tblock); }
+int set_tile(afx_sim* s, int tile) { return set_tile_tb(s, tile, s->tblock, s->xcd_rotate); }
 
 bool can_pair(const afx_sim* s, int tile)
 {
@@ -2098,7 +2108,12 @@ struct TuneKey {
 };
 // Results shared by every handle of the process (handles may be driven from
 // different threads, so lookups and inserts hold the mutex; a trial does not).
-std::map<TuneKey, std::pair<int, int>> g_tune_cache;  // -> (tile, steps per pass)
+struct TuneChoice {
+    int tile, tblock;
+    bool rotate;
+    bool operator!=(const TuneChoice& o) const { return tile != o.tile || tblock != o.tblock || rotate != o.rotate; }
+};
+std::map<TuneKey, TuneChoice> g_tune_cache;
 std::mutex g_tune_mutex;
 
 }  // namespace
@@ -2129,7 +2144,7 @@ extern "C" int afx_set_time_block(afx_sim* s, int32_t steps)
     }
     HIP_TRY(hipSetDevice(s->cfg.device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    return set_tile_tb(s, s->tile, steps);
+    return set_tile_tb(s, s->tile, steps, s->xcd_rotate);
 }
 
 extern "C" int afx_get_time_block(afx_sim* s, int32_t* steps, int64_t* pair_tiles)
@@ -2159,7 +2174,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
     key.flags = s->cfg.flags;
     key.device = s->cfg.device;
     bool cached = false;
-    std::pair<int, int> choice{};
+    TuneChoice choice{};
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto hit = g_tune_cache.find(key);
@@ -2168,22 +2183,27 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
             choice = hit->second;
         }
     }
+    const TuneChoice current{s->tile, s->tblock, s->xcd_rotate};
     if (cached) {
-        if (choice.first != s->tile || choice.second != s->tblock)
-            if (int rc = set_tile_tb(s, choice.first, choice.second)) return rc;
+        if (choice != current)
+            if (int rc = set_tile_tb(s, choice.tile, choice.tblock, choice.rotate)) return rc;
     } else {
         Snapshot sn;
         if (int rc = snap_take(s, sn)) return rc;
         // candidates: every tile shape, with one and (where the fused path
-        // allows it) two steps per pass; the current setting first, as the
-        // reference of the bit-exactness check
-        std::vector<std::pair<int, int>> cands = {{s->tile, s->tblock}};
+        // allows it) two steps per pass, each with and without the XCD layer
+        // rotation; the current setting first, as the reference of the
+        // bit-exactness check
+        std::vector<TuneChoice> cands = {current};
         for (int tl = 0; tl < kNumTiles; ++tl)
             for (int tb = 1; tb <= (can_pair(s, tl) ? 2 : 1); ++tb)
-                if (tl != s->tile || tb != s->tblock) cands.push_back({tl, tb});
+                for (int ro = 1; ro >= 0; --ro) {
+                    const TuneChoice c{tl, tb, ro != 0};
+                    if (c != current) cands.push_back(c);
+                }
         const size_t fbytes = s->P[0].bytes;
         std::vector<char> ref(fbytes), got(fbytes);
-        std::pair<int, int> best = cands[0];
+        TuneChoice best = cands[0];
         float best_ms = 1e30f;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
@@ -2191,7 +2211,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         for (size_t ci = 0; ci < cands.size(); ++ci) {
             const bool is_ref = ci == 0;
             if (int rc = snap_restore(s, sn)) return rc;
-            if (int rc = set_tile_tb(s, cands[ci].first, cands[ci].second)) return rc;
+            if (int rc = set_tile_tb(s, cands[ci].tile, cands[ci].tblock, cands[ci].rotate)) return rc;
             // untimed pass (captures graphs, warms caches), then three timed
             // passes from the same state; the first is also the check pass
             if (int rc = run_steps(s, steps_per_trial)) return rc;
@@ -2218,7 +2238,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (int rc = snap_restore(s, sn)) return rc;
-        if (int rc = set_tile_tb(s, best.first, best.second)) return rc;
+        if (int rc = set_tile_tb(s, best.tile, best.tblock, best.rotate)) return rc;
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         g_tune_cache[key] = best;
     }

@@ -200,11 +200,12 @@ int afx_set_time_block(afx_sim* sim, int32_t steps);
 int afx_get_time_block(afx_sim* sim, int32_t* steps, int64_t* pair_tiles);
 
 /* Times the candidate interior tile shapes, each with one and two steps per
- * pass where that applies, on this handle's rooms (each is first checked
- * bit-exact against the current setting on a copy of the state; the best of
- * three timed passes counts), keeps the fastest and returns its tile in
- * tile[3] (afx_get_time_block gives its steps per pass).  Leaves the
- * simulation state unchanged. */
+ * pass where that applies and with two assignments of tile layers to the
+ * XCDs, on this handle's rooms (each is first checked bit-exact against the
+ * current setting on a copy of the state; the best of three timed passes
+ * counts), keeps the fastest and returns its tile in tile[3]
+ * (afx_get_time_block gives its steps per pass).  Leaves the simulation
+ * state unchanged. */
 int afx_autotune(afx_sim* sim, int32_t steps_per_trial, int32_t tile[3]);
 
 /* Measurement aid, no counterpart in the reference: a plain device copy of
