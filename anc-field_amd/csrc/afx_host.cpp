@@ -245,6 +245,9 @@ int check_sim(afx_sim* s)
 
 // Classify the air cells of one room (lrs-data.hpp:117-215): cells with
 // 1 <= x <= nx-2 (likewise y, z) whose 6-neighbourhood holds solid voxels.
+// Compat mode scans only the voxel volume, 1 <= x <= vx-2, as LrsData does
+// (lrs-data.hpp:101-105,154-158): when minsize grows the grid beyond the
+// model, cells on or past the volume's last plane stay plain leapfrog cells.
 int classify_volume(const Room& r, int ridx, bool compat, std::vector<Entry>& out, int64_t* group_count)
 {
     if (r.vox.empty()) return AFX_OK;
@@ -253,9 +256,12 @@ int classify_volume(const Room& r, int ridx, bool compat, std::vector<Entry>& ou
         if (x >= r.vx || y >= r.vy || z >= r.vz) return -1;  // air padding beyond the voxel volume
         return r.vox[(size_t)((z * r.vy + y) * r.vx + x)];
     };
-    for (int64_t z = 1; z < r.nz - 1; ++z)
-        for (int64_t y = 1; y < r.ny - 1; ++y)
-            for (int64_t x = 1; x < r.nx - 1; ++x) {
+    const int64_t lx = compat ? std::min(r.nx, r.vx) : r.nx;
+    const int64_t ly = compat ? std::min(r.ny, r.vy) : r.ny;
+    const int64_t lz = compat ? std::min(r.nz, r.vz) : r.nz;
+    for (int64_t z = 1; z < lz - 1; ++z)
+        for (int64_t y = 1; y < ly - 1; ++y)
+            for (int64_t x = 1; x < lx - 1; ++x) {
                 if (V(x, y, z) != -1) continue;
                 const int32_t nb6[6] = {V(x + 1, y, z), V(x - 1, y, z), V(x, y + 1, z),
                                         V(x, y - 1, z), V(x, y, z + 1), V(x, y, z - 1)};
@@ -816,8 +822,12 @@ int build_special(afx_sim* s)
     s->src_cap = cap;
     HIP_TRY(upload(s->d_src, sbuf, s->stream));
     HIP_TRY(upload(s->d_src_len, slen, s->stream));
-    // FP16/BF16 storage scale: a power of two at the sources' peak amplitude
-    if (s->prec == Prec::F16 || s->prec == Prec::BF16) {
+    // FP16/BF16 storage scale: a power of two at the sources' peak amplitude.
+    // It is fixed by the first run: fields, IIR rings and PML state hold
+    // values in units of pressure / scale, so a source added later keeps it.
+    if ((s->prec == Prec::F16 || s->prec == Prec::BF16) && s->t > 0) {
+        // keep s->scale
+    } else if (s->prec == Prec::F16 || s->prec == Prec::BF16) {
         if (s->cfg.scale > 0) {
             s->scale = s->cfg.scale;
         } else {

@@ -84,7 +84,8 @@ typedef struct afx_config {
     int32_t device;     /* HIP device ordinal */
     float pml_order;    /* physical PML: polynomial grading order (default 3) */
     float pml_r0;       /* physical PML: design normal-incidence reflection (default 1e-3) */
-    float scale;        /* FP16/BF16 storage: power-of-two field scale (0: chosen from the sources) */
+    float scale;        /* FP16/BF16 storage: power-of-two field scale (0: the smallest power of two at or
+                           above the peak of the sources added before the first afx_run; fixed from then on) */
 } afx_config;
 
 typedef struct afx_stats {

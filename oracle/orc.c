@@ -31,12 +31,25 @@ static int orc_group_of(int pattern)
 int64_t orc_classify(int32_t nx, int32_t ny, int32_t nz, const int32_t* v,
                      int32_t* pos, int32_t* group, int64_t cap)
 {
+    return orc_classify_range(nx, ny, nz, v, nx, ny, nz, pos, group, cap);
+}
+
+/* The scan of lrs-data.hpp:154-158 runs over 1 <= x < lim_x - 1 (likewise y,
+ * z), where lim is the voxel volume's size (lrs-data.hpp:101-105): a grid
+ * grown beyond the volume by minsize keeps plain leapfrog cells there. */
+int64_t orc_classify_range(int32_t nx, int32_t ny, int32_t nz, const int32_t* v,
+                           int32_t lx, int32_t ly, int32_t lz,
+                           int32_t* pos, int32_t* group, int64_t cap)
+{
     const int64_t sx = nx, sxy = (int64_t)nx * ny;
+    if (lx > nx) lx = nx;
+    if (ly > ny) ly = ny;
+    if (lz > nz) lz = nz;
     int64_t count[26] = {0};
     /* pass 1: counts per group (and the unphysical check) */
-    for (int64_t z = 1; z < nz - 1; ++z)
-        for (int64_t y = 1; y < ny - 1; ++y)
-            for (int64_t x = 1; x < nx - 1; ++x) {
+    for (int64_t z = 1; z < lz - 1; ++z)
+        for (int64_t y = 1; y < ly - 1; ++y)
+            for (int64_t x = 1; x < lx - 1; ++x) {
                 const int64_t i = z * sxy + y * sx + x;
                 if (v[i] != -1) continue;
                 int pattern = 0;
@@ -60,9 +73,9 @@ int64_t orc_classify(int32_t nx, int32_t ny, int32_t nz, const int32_t* v,
     if (off[26] > cap) return off[26];
     int64_t fill[26];
     for (int g = 0; g < 26; ++g) fill[g] = off[g];
-    for (int64_t z = 1; z < nz - 1; ++z)
-        for (int64_t y = 1; y < ny - 1; ++y)
-            for (int64_t x = 1; x < nx - 1; ++x) {
+    for (int64_t z = 1; z < lz - 1; ++z)
+        for (int64_t y = 1; y < ly - 1; ++y)
+            for (int64_t x = 1; x < lx - 1; ++x) {
                 const int64_t i = z * sxy + y * sx + x;
                 if (v[i] != -1) continue;
                 int mats[6] = {-1, -1, -1, -1, -1, -1};

@@ -83,7 +83,10 @@ struct SIM* FN(orc_create)(const orc_problem* pb)
         int64_t cap = s->n;
         int32_t* pos = (int32_t*)malloc((size_t)cap * 6 * sizeof(int32_t));
         int32_t* grp = (int32_t*)malloc((size_t)cap * sizeof(int32_t));
-        int64_t nb = orc_classify(pb->nx, pb->ny, pb->nz, pb->vox, pos, grp, cap);
+        const int32_t lx = pb->vol[0] > 0 ? pb->vol[0] : pb->nx;
+        const int32_t ly = pb->vol[1] > 0 ? pb->vol[1] : pb->ny;
+        const int32_t lz = pb->vol[2] > 0 ? pb->vol[2] : pb->nz;
+        int64_t nb = orc_classify_range(pb->nx, pb->ny, pb->nz, pb->vox, lx, ly, lz, pos, grp, cap);
         if (nb < 0) { free(pos); free(grp); FN(orc_destroy)(s); return NULL; }
         s->nb = nb; s->bpos = pos; s->bgroup = grp;
         int64_t nr = nb * 3 * (pb->ncoef > 0 ? pb->ncoef : 1);
@@ -471,7 +474,14 @@ void FN(orc_field)(const struct SIM* s, int which, REAL* out)
     memcpy(out, s->buf[s->rot[which]], (size_t)s->n * sizeof(REAL));
 }
 
+void FN(orc_set_field)(struct SIM* s, int which, const REAL* in)
+{
+    memcpy(s->buf[s->rot[which]], in, (size_t)s->n * sizeof(REAL));
+}
+
 int64_t FN(orc_nbound)(const struct SIM* s) { return s->nb; }
+
+int32_t FN(orc_ring_index)(const struct SIM* s) { return s->ix_buf; }
 
 void FN(orc_rings)(const struct SIM* s, REAL* in, REAL* out)
 {

@@ -31,13 +31,16 @@ class Problem(C.Structure):
         ("ksi", _f64p * 3), ("ksi_h", _f64p * 3),
         ("pml_boxes", _i32p), ("pml_nbox", C.c_int32),
         ("store", C.c_int32), ("scale", C.c_double),
+        ("vol", C.c_int32 * 3),
     ]
 
 
-def _load():
-    if not os.path.exists(ORC_PATH):
+def load_lib(path=ORC_PATH):
+    """The oracle library at `path` with its argument types declared (a
+    variant built from edited sources loads the same way)."""
+    if path == ORC_PATH and not os.path.exists(ORC_PATH):
         subprocess.run([os.path.join(ROOT, "oracle", "build.sh")], check=True)
-    lib = C.CDLL(ORC_PATH)
+    lib = C.CDLL(path)
     P = C.c_void_p
     for sfx, T in (("f32", C.c_float), ("f64", C.c_double)):
         Tp = C.POINTER(T)
@@ -46,11 +49,17 @@ def _load():
         getattr(lib, f"orc_run_{sfx}").restype = C.c_int
         getattr(lib, f"orc_run_{sfx}").argtypes = [P, C.c_int64, Tp]
         getattr(lib, f"orc_field_{sfx}").argtypes = [P, C.c_int, Tp]
+        getattr(lib, f"orc_set_field_{sfx}").argtypes = [P, C.c_int, Tp]
         getattr(lib, f"orc_destroy_{sfx}").argtypes = [P]
         getattr(lib, f"orc_nbound_{sfx}").restype = C.c_int64
         getattr(lib, f"orc_nbound_{sfx}").argtypes = [P]
         getattr(lib, f"orc_rings_{sfx}").argtypes = [P, Tp, Tp]
+        getattr(lib, f"orc_ring_index_{sfx}").restype = C.c_int32
+        getattr(lib, f"orc_ring_index_{sfx}").argtypes = [P]
     lib.orc_classify.restype = C.c_int64
+    lib.orc_classify_range.restype = C.c_int64
+    lib.orc_classify_range.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, C.c_int32, C.c_int32, C.c_int32,
+                                       _i32p, _i32p, C.c_int64]
     lib.orc_classify.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p, _i32p, C.c_int64]
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_round_store.restype = C.c_float
@@ -58,7 +67,7 @@ def _load():
     return lib
 
 
-lib = _load()
+lib = load_lib()
 
 STORE_MODES = {None: 0, "fp16": 1, "bf16": 2}
 
@@ -77,14 +86,18 @@ def storage_scale(src):
     return math.ldexp(1.0, math.ceil(math.log2(mx))) if mx > 0 else 1.0
 
 
-def classify(vox):
-    """Reference-order boundary list: (pos[n,6], group[n]) or raises on an unphysical pattern."""
+def classify(vox, vol=None):
+    """Reference-order boundary list: (pos[n,6], group[n]) or raises on an
+    unphysical pattern.  vol = (vx, vy, vz) limits the scan to that voxel
+    volume inside the grid."""
     v = np.ascontiguousarray(vox, np.int32)
     nz, ny, nx = v.shape
     cap = max(int(v.size), 1)
     pos = np.zeros((cap, 6), np.int32)
     grp = np.zeros(cap, np.int32)
-    n = lib.orc_classify(nx, ny, nz, v.ctypes.data_as(_i32p), pos.ctypes.data_as(_i32p), grp.ctypes.data_as(_i32p), cap)
+    lx, ly, lz = vol if vol is not None else (nx, ny, nz)
+    n = lib.orc_classify_range(nx, ny, nz, v.ctypes.data_as(_i32p), lx, ly, lz, pos.ctypes.data_as(_i32p),
+                               grp.ctypes.data_as(_i32p), cap)
     if n < 0:
         raise ValueError(f"unphysical pattern at cell {tuple(pos[0, :3])}")
     return pos[:n].copy(), grp[:n].copy()
@@ -178,7 +191,8 @@ class Oracle:
 
     def __init__(self, vox, a, b, c, dx, dt, compat, src_cells=(), src=None, mic_cells=(), precision="f32",
                  pml_mode=0, pml_width=0, pml_faces=0, pml_profile=None, dims=None, threads=8,
-                 storage=None, scale=None):
+                 storage=None, scale=None, vol=None, lib=None, compat_kappa=None):
+        self.lib = lib = lib if lib is not None else globals()["lib"]
         self.sfx = precision
         self.T = np.float32 if precision == "f32" else np.float64
         self._keep = []
@@ -220,6 +234,9 @@ class Oracle:
         pb.pml_width = pml_width
         if pml_mode == 1:
             ks = compat_ksi((nx, ny, nz), [] if pml_profile is None else list(np.asarray(pml_profile, np.float32)))
+            if compat_kappa is not None:  # explicit per-axis kappa arrays (x, y, z)
+                ks = [np.ascontiguousarray(np.pad(np.asarray(k, np.float64), (0, max(nx, ny, nz) + 2 - len(k))))
+                      for k in compat_kappa]
             for d in range(3):
                 pb.ksi[d] = ks[d].ctypes.data_as(_f64p)
             self._keep += ks
@@ -237,6 +254,8 @@ class Oracle:
         pb.store = STORE_MODES[storage]
         self.scale = float(scale) if scale else (storage_scale(src) if storage else 1.0)
         pb.scale = self.scale
+        if vol is not None:
+            pb.vol[0], pb.vol[1], pb.vol[2] = vol
         self._pb = pb
         lib.orc_set_threads(threads)
         self._h = getattr(lib, f"orc_create_{self.sfx}")(C.byref(pb))
@@ -247,7 +266,7 @@ class Oracle:
     def run(self, steps):
         tr = np.zeros((max(self.nmic, 1), steps), self.T)
         ptr = tr.ctypes.data_as(C.POINTER(C.c_float if self.T == np.float32 else C.c_double))
-        getattr(lib, f"orc_run_{self.sfx}")(self._h, steps, ptr)
+        getattr(self.lib, f"orc_run_{self.sfx}")(self._h, steps, ptr)
         self.traces.append(tr[:self.nmic])
         return tr[:self.nmic]
 
@@ -257,15 +276,37 @@ class Oracle:
     def field(self, which=0):
         out = np.zeros(self.dims, self.T)
         ptr = out.ctypes.data_as(C.POINTER(C.c_float if self.T == np.float32 else C.c_double))
-        getattr(lib, f"orc_field_{self.sfx}")(self._h, which, ptr)
+        getattr(self.lib, f"orc_field_{self.sfx}")(self._h, which, ptr)
         return out * self.T(self.scale)  # pressure; the scale is a power of two
 
+    def set_field(self, which, values):
+        """Overwrites pressure field `which` (0 newest, 1 p^n, 2 p^{n-1}) with
+        `values` [nz][ny][nx] in pressure units."""
+        v = np.ascontiguousarray(np.asarray(values, np.float64) / self.scale, self.T)
+        assert v.shape == self.dims
+        ptr = v.ctypes.data_as(C.POINTER(C.c_float if self.T == np.float32 else C.c_double))
+        getattr(self.lib, f"orc_set_field_{self.sfx}")(self._h, which, ptr)
+
     def nbound(self):
-        return getattr(lib, f"orc_nbound_{self.sfx}")(self._h)
+        return getattr(self.lib, f"orc_nbound_{self.sfx}")(self._h)
+
+    def rings(self):
+        """IIR state (ring_in, ring_out), each [nb][3*ncoef], boundary cells in classify order."""
+        nb = self.nbound()
+        n3 = 3 * max(int(self._pb.ncoef), 1)
+        rin = np.zeros((nb, n3), self.T)
+        rout = np.zeros((nb, n3), self.T)
+        P = C.POINTER(C.c_float if self.T == np.float32 else C.c_double)
+        getattr(self.lib, f"orc_rings_{self.sfx}")(self._h, rin.ctypes.data_as(P), rout.ctypes.data_as(P))
+        return rin, rout
+
+    def ring_index(self):
+        """Ring slot the last step wrote."""
+        return int(getattr(self.lib, f"orc_ring_index_{self.sfx}")(self._h))
 
     def close(self):
         if self._h:
-            getattr(lib, f"orc_destroy_{self.sfx}")(self._h)
+            getattr(self.lib, f"orc_destroy_{self.sfx}")(self._h)
             self._h = None
 
     def __del__(self):

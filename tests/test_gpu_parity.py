@@ -383,12 +383,22 @@ def test_shoebox_small_full_run_bitexact():
 
 def test_furnished_room_bitexact():
     """The benchmark room itself (256x192x128, 12 boxes, 4 IIR materials,
-    12-cell PML on x+, 16 receivers) for 60 steps: identical to the oracle."""
-    _case("furnished 256x192x128, 60 steps")
-    w = rooms.furnished(steps=60)
-    got, want, f, wf = _workload_pair(w, 60)
+    12-cell PML on x+, 16 receivers) for 800 steps: identical to the oracle.
+    The wave travels 0.57 cells per step, so after 800 steps it has crossed
+    the whole room (the farthest cell is 244 cells from the source): the
+    x+ PML slab, every box's boundary cells and every receiver carry it."""
+    _case("furnished 256x192x128, 800 steps")
+    steps = 800
+    w = rooms.furnished(steps=steps)
+    got, want, f, wf = _workload_pair(w, steps)
     _assert_bitexact(got, want, "traces")
     _assert_bitexact(f, wf, "field")
+    nz, ny, nx = w.vox.shape
+    assert np.all(np.abs(want).max(axis=1) > 0), "a receiver was not reached"
+    assert np.abs(wf[:, :, nx - 1 - w.pml_width:nx - 1]).max() > 0, "the PML slab was not reached"
+    pos, grp = orc.classify(w.vox)
+    bvals = wf[pos[:, 2], pos[:, 1], pos[:, 0]]
+    assert np.count_nonzero(bvals) > 0.99 * len(bvals), "boundary cells not reached"
 
 
 def test_large_room_linearity():

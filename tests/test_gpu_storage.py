@@ -131,3 +131,45 @@ def test_fp16_long_tail_matches_oracle():
     sub = np.count_nonzero((np.abs(wf / scale) < 2.0**-14) & (wf != 0))
     print(f"cells in the subnormal range: {sub}")
     assert sub > 0
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_source_and_receiver_added_mid_run(storage):
+    """A louder source and a receiver added after the first run: the storage
+    scale stays the one the first run fixed (fields, rings and PML state are
+    held in its units), the new source enters at its own sample index and the
+    new receiver records from then on.  The oracle holds both sources from
+    the start (the late one silent until then)."""
+    _case(f"{storage}: source + receiver added after 100 of 220 steps")
+    v, a, b, rng = _room(86)
+    inner = rooms.interior_cells(v, rng, 5)
+    steps, t1 = 220, 100
+    dt = rooms.courant_dt(DX)
+    first = rooms.ricker(steps, dt, 1000.0, amp=0.9)
+    late = rooms.ricker(steps, dt, 1300.0, amp=6.0, t0=t1 * dt + 1.2 / 1300.0)
+    late[:t1] = 0.0
+    prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
+    s = afx.Sim(c=rooms.C_AIR, dx=DX, dt=dt, res=DX, precision=storage, compat=False)
+    s.set_voxels(v)
+    s.set_materials(a, b)
+    s.set_pml(6, rooms.FACE_BITS["x+"], prof)
+    s.add_source_cell(inner[0], first)
+    s.add_receiver_cell(inner[1])
+    s.add_receiver_cell(inner[2])
+    s.run(t1)
+    s.add_source_cell(inner[3], late)
+    s.add_receiver_cell(inner[4])
+    s.run(steps - t1)
+    got, field = s.receivers(), s.pressure(0)
+    s.close()
+    o = orc.Oracle(v, a, b, rooms.C_AIR, DX, dt, False, np.stack([inner[0], inner[3]]), np.stack([first, late]),
+                   inner[[1, 2, 4]], "f32", 2, 6, rooms.FACE_BITS["x+"], prof,
+                   storage=None if storage == "fp32" else storage,
+                   scale=None if storage == "fp32" else orc.storage_scale(first))
+    o.run(steps)
+    want = o.receivers()
+    want[2, :t1] = 0.0  # the late receiver records from its first step on
+    assert np.abs(want[2, t1:]).max() > 1.0  # the late source is louder than the first scale
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(field, o.field(0), "field")
+    o.close()

@@ -242,3 +242,34 @@ def test_fp32_tracks_fp64():
     o64 = orc.Oracle(v, a, b, C0, DX, DT, False, [[5, 5, 5]], sig[None], [[14, 12, 9]], precision="f64")
     x, y = o32.run(400), o64.run(400)
     assert np.linalg.norm(x - y) / np.linalg.norm(y) < 1e-4
+
+
+def test_classifier_scan_range_brute_force():
+    """The scan of lrs-data.hpp:154-158 runs over 1 <= x < vx-1 of the voxel
+    volume (lrs-data.hpp:101-105).  Brute force in numpy over a volume padded
+    into a larger grid: the boundary cells are exactly the air cells inside
+    that range with a solid neighbour, and the whole-grid scan adds the cells
+    outside it."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    vx, vy, vz = 18, 16, 14
+    v = rooms.repair(rooms.add_boxes(rooms.shoebox(vx, vy, vz, [0, 1, 0, 1, 0, 1]), rng, 4, 2, 5, 2))
+    grid = np.full((20, 22, 25), -1, np.int32)
+    grid[:vz, :vy, :vx] = v
+
+    def brute(lim):
+        s = grid != -1
+        cells = set()
+        for z in range(1, lim[2] - 1):
+            for y in range(1, lim[1] - 1):
+                for x in range(1, lim[0] - 1):
+                    if s[z, y, x]:
+                        continue
+                    if s[z, y, x + 1] or s[z, y, x - 1] or s[z, y + 1, x] or s[z, y - 1, x] or s[z + 1, y, x] \
+                            or s[z - 1, y, x]:
+                        cells.add((x, y, z))
+        return cells
+
+    for lim in ((vx, vy, vz), (25, 22, 20)):
+        pos, grp = orc.classify(grid, vol=lim)
+        assert {tuple(p[:3]) for p in pos} == brute(lim)
+    assert len(orc.classify(grid)[1]) > len(orc.classify(grid, vol=(vx, vy, vz))[1])
