@@ -388,6 +388,7 @@ _MLP_SIGS = {
     "afx_mlp_gradients": (C.c_int, [_P, _f32p, _f32p, _f32p, C.c_int64, _f32p, C.POINTER(C.c_double), _f32p]),
     "afx_mlp_set_data": (C.c_int, [_P, _f32p, _f32p, _f32p, C.c_int64]),
     "afx_mlp_train": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, C.c_uint32, _f32p]),
+    "afx_mlp_train_noalloc": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, C.c_uint32, _f32p]),
     "afx_mlp_get_stats": (C.c_int, [_P, C.POINTER(MlpStats)]),
 }
 for _name, (_res, _args) in _MLP_SIGS.items():
@@ -480,6 +481,12 @@ class Mlp:
     def train(self, steps, lr0, lr1=0.0, seed=0):
         loss = np.empty(int(steps), np.float32)
         _check(lib.afx_mlp_train(self._h, int(steps), lr0, lr1, seed, loss.ctypes.data_as(_f32p)))
+        return loss
+
+    def train_noalloc(self, steps, lr0, lr1=0.0, seed=0):
+        """train() without device allocation (fixed per-GPU scratch, eager steps)."""
+        loss = np.empty(int(steps), np.float32)
+        _check(lib.afx_mlp_train_noalloc(self._h, int(steps), lr0, lr1, seed, loss.ctypes.data_as(_f32p)))
         return loss
 
     def stats(self):

@@ -43,6 +43,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <random>
@@ -621,6 +622,11 @@ __global__ void k_step_inc(int64_t* step, int64_t k)
     if (threadIdx.x == 0) *step += k;
 }
 
+__global__ void k_step_zero(int64_t* step)
+{
+    if (threadIdx.x == 0) *step = 0;
+}
+
 #define HIP_CHECK(expr)                                                                                  \
     do {                                                                                                 \
         hipError_t e_ = (expr);                                                                          \
@@ -657,6 +663,13 @@ struct Arena {
 
 }  // namespace
 
+// Device memory of afx_mlp_train_noalloc: minibatch indices, learning-rate
+// schedule, loss partials and the step counter of one call, laid out as the
+// buffers afx_mlp_train allocates per call.  Each GPU holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (scratch_setup).
+static __device__ __align__(256) unsigned char scratch[AFX_MLP_TRAIN_SCRATCH_BYTES];
+
 struct afx_mlp {
     afx_mlp_config cfg{};
     int device = 0;
@@ -690,6 +703,8 @@ struct afx_mlp {
     // training set
     float *xd = nullptr, *td = nullptr, *wd = nullptr;
     int64_t ndata = 0;
+    std::vector<int> perm;          // host permutation of the training rows (sized by set_data)
+    unsigned char* scratch_dev = nullptr;  // this device's `scratch` (device address)
     int64_t t = 0;                  // optimizer steps taken
     afx_mlp_stats st{};
     std::vector<float> freq_host;
@@ -897,6 +912,71 @@ int set_device(afx_mlp* s)
     return AFX_OK;
 }
 
+// Device address of `scratch` on each device, looked up once per device
+// with that device current.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxScratchDevices] = {};
+
+int scratch_setup(afx_mlp* s)
+{
+    if (s->device >= kMaxScratchDevices) return err(AFX_E_RANGE, "device %d above %d", s->device, kMaxScratchDevices - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[s->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return err(AFX_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", s->device, hipGetErrorString(e));
+        g_scratch_dev[s->device] = static_cast<unsigned char*>(p);
+    }
+    s->scratch_dev = g_scratch_dev[s->device];
+    return AFX_OK;
+}
+
+// Byte offsets of one training call's buffers (each rounded up to 256):
+// minibatch indices [steps][B] int32, loss partials [steps][nblk_loss],
+// schedule [steps][3] FP32, step counter int64.  Returns the total.
+size_t train_layout(const afx_mlp* s, int64_t steps, size_t off[4])
+{
+    auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
+    off[0] = 0;
+    off[1] = off[0] + up((size_t)steps * s->B * sizeof(int));
+    off[2] = off[1] + up((size_t)steps * s->nblk_loss * sizeof(float));
+    off[3] = off[2] + up((size_t)steps * 3 * sizeof(float));
+    return off[3] + 256;
+}
+
+// Minibatch rows (a fresh seeded permutation per epoch, consumed B at a
+// time) and, per step, the cosine learning rate from lr0 to lr1 and Adam's
+// bias corrections, into idx[steps*B] and sched[steps*3].
+void make_schedule(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed, int* idx, float* sched)
+{
+    std::vector<int>& perm = s->perm;
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    size_t pos = perm.size();
+    const size_t n = (size_t)steps * s->B;
+    for (size_t i = 0; i < n; ++i) {
+        if (pos == perm.size()) {
+            std::shuffle(perm.begin(), perm.end(), rng);
+            pos = 0;
+        }
+        idx[i] = perm[pos++];
+    }
+    const int64_t t0 = s->t;
+    for (int64_t k = 0; k < steps; ++k) {
+        const double frac = steps > 1 ? (double)k / (double)(steps - 1) : 1.0;
+        sched[(size_t)(3 * k)] = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(M_PI * frac)));
+        sched[(size_t)(3 * k + 1)] = (float)(1.0 - std::pow((double)s->cfg.beta1, (double)(t0 + k + 1)));
+        sched[(size_t)(3 * k + 2)] = (float)(1.0 - std::pow((double)s->cfg.beta2, (double)(t0 + k + 1)));
+    }
+}
+
+// Host image of the scratch layout for afx_mlp_train_noalloc (static, so the
+// call allocates nothing); guarded by g_stage_mu for the whole call.
+std::mutex g_stage_mu;
+alignas(256) unsigned char g_stage[AFX_MLP_TRAIN_SCRATCH_BYTES];
+
 void init_params(const afx_mlp* s, std::vector<float>& p)
 {
     std::mt19937_64 rng(s->cfg.seed);
@@ -1078,6 +1158,7 @@ int afx_mlp_create(const afx_mlp_config* cfg, afx_mlp** out)
     if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
         hipEventCreate(&s->ev1) != hipSuccess)
         return fail_free(err(AFX_E_HIP, "stream/event creation failed"));
+    if (int rc = scratch_setup(s)) return fail_free(rc);
 
     const int B = s->B;
     s->nblk_loss = (B / 32) * (c.out_dim / 32);
@@ -1263,6 +1344,7 @@ int afx_mlp_set_data(afx_mlp* s, const float* x, const float* target, const floa
     HIP_CHECK(hipMemcpyAsync(s->td, target, (size_t)rows * O * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_CHECK(hipMemcpyAsync(s->wd, weight, (size_t)rows * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->perm.assign((size_t)rows, 0);
     s->ndata = rows;
     return AFX_OK;
 }
@@ -1275,28 +1357,10 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
     if (!(lr0 >= 0) || !(lr1 >= 0)) return err(AFX_E_ARG, "learning rates must be >= 0");
     if (int rc = set_device(s)) return rc;
     const int B = s->B;
-    // minibatch index stream: a fresh permutation per epoch, consumed B at a time
     std::vector<int> idx((size_t)steps * B);
-    std::vector<int> perm((size_t)s->ndata);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::mt19937_64 rng(seed);
-    size_t pos = perm.size();
-    for (size_t i = 0; i < idx.size(); ++i) {
-        if (pos == perm.size()) {
-            std::shuffle(perm.begin(), perm.end(), rng);
-            pos = 0;
-        }
-        idx[i] = perm[pos++];
-    }
-    // per step: cosine learning rate from lr0 to lr1 and Adam's bias corrections
     std::vector<float> sched((size_t)steps * 3);
+    make_schedule(s, steps, lr0, lr1, seed, idx.data(), sched.data());
     const int64_t t0 = s->t;
-    for (int64_t k = 0; k < steps; ++k) {
-        const double frac = steps > 1 ? (double)k / (double)(steps - 1) : 1.0;
-        sched[(size_t)(3 * k)] = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(M_PI * frac)));
-        sched[(size_t)(3 * k + 1)] = (float)(1.0 - std::pow((double)s->cfg.beta1, (double)(t0 + k + 1)));
-        sched[(size_t)(3 * k + 2)] = (float)(1.0 - std::pow((double)s->cfg.beta2, (double)(t0 + k + 1)));
-    }
     int* d_idx = nullptr;
     float *d_part = nullptr, *d_sched = nullptr;
     int64_t* d_step = nullptr;
@@ -1378,6 +1442,73 @@ int afx_mlp_train(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed
         }
     }
     return cleanup(AFX_OK);
+}
+
+int afx_mlp_train_noalloc(afx_mlp* s, int64_t steps, float lr0, float lr1, uint32_t seed, float* loss_out)
+{
+    if (!s) return err(AFX_E_ARG, "null handle");
+    if (steps < 1) return err(AFX_E_ARG, "steps must be >= 1");
+    if (!s->xd) return err(AFX_E_STATE, "no training set (afx_mlp_set_data)");
+    if (!(lr0 >= 0) || !(lr1 >= 0)) return err(AFX_E_ARG, "learning rates must be >= 0");
+    if (!s->scratch_dev) return err(AFX_E_STATE, "training scratch not set up on device %d", s->device);
+    size_t off[4];
+    if (steps > (int64_t)AFX_MLP_TRAIN_SCRATCH_BYTES)
+        return err(AFX_E_RANGE, "%lld steps exceed the %u-byte training scratch", (long long)steps,
+                   (unsigned)AFX_MLP_TRAIN_SCRATCH_BYTES);
+    const size_t need = train_layout(s, steps, off);
+    if (need > AFX_MLP_TRAIN_SCRATCH_BYTES)
+        return err(AFX_E_RANGE,
+                   "%lld steps at batch %d need %zu bytes of training scratch, above its %u bytes: split the call",
+                   (long long)steps, s->B, need, (unsigned)AFX_MLP_TRAIN_SCRATCH_BYTES);
+    if (int rc = set_device(s)) return rc;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    make_schedule(s, steps, lr0, lr1, seed, reinterpret_cast<int*>(g_stage + off[0]),
+                  reinterpret_cast<float*>(g_stage + off[2]));
+    unsigned char* base = s->scratch_dev;
+    int* d_idx = reinterpret_cast<int*>(base + off[0]);
+    float* d_part = reinterpret_cast<float*>(base + off[1]);
+    float* d_sched = reinterpret_cast<float*>(base + off[2]);
+    int64_t* d_step = reinterpret_cast<int64_t*>(base + off[3]);
+    const int64_t t0 = s->t;
+    float* keep_partial = s->partial;
+    auto finish = [&](int rc) {
+        (void)hipStreamSynchronize(s->stream);
+        s->partial = keep_partial;
+        s->sr = StepRef{};
+        return rc;
+    };
+    if (hipMemcpyAsync(d_idx, g_stage + off[0], off[1] - off[0], hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(d_sched, g_stage + off[2], off[3] - off[2], hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return finish(err(AFX_E_HIP, "training schedule upload failed"));
+    k_step_zero<<<1, 64, 0, s->stream>>>(d_step);
+    if (hipGetLastError() != hipSuccess) return finish(err(AFX_E_HIP, "step counter launch failed"));
+    s->partial = d_part;
+    s->sr = StepRef{d_step, (int64_t)s->B, (int64_t)s->nblk_loss, d_sched};
+    const auto w0 = std::chrono::steady_clock::now();
+    if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return finish(err(AFX_E_HIP, "event record failed"));
+    for (int64_t k = 0; k < steps; ++k) {
+        s->sr.rel = k;
+        if (int rc = step_grad(s, s->xd, s->td, s->wd, d_idx, nullptr, true)) return finish(rc);
+    }
+    if (hipEventRecord(s->ev1, s->stream) != hipSuccess) return finish(err(AFX_E_HIP, "event record failed"));
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return finish(err(AFX_E_HIP, "training failed on the device"));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    s->t = t0 + steps;
+    s->st.last_train_gpu_ms = ms;
+    s->st.last_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    s->st.train_steps = s->t;
+    if (loss_out) {
+        float* part = reinterpret_cast<float*>(g_stage + off[1]);
+        if (hipMemcpy(part, d_part, (size_t)steps * s->nblk_loss * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            return finish(err(AFX_E_HIP, "loss readback failed"));
+        for (int64_t k = 0; k < steps; ++k) {
+            double acc = 0.0;
+            for (int b = 0; b < s->nblk_loss; ++b) acc += part[(size_t)k * s->nblk_loss + b];
+            loss_out[k] = (float)acc;
+        }
+    }
+    return finish(AFX_OK);
 }
 
 int afx_mlp_get_stats(afx_mlp* s, afx_mlp_stats* st)

@@ -104,6 +104,22 @@ int afx_mlp_set_data(afx_mlp* mlp, const float* x, const float* target, const fl
  * given). */
 int afx_mlp_train(afx_mlp* mlp, int64_t steps, float lr0, float lr1, uint32_t seed, float* loss_out);
 
+/* afx_mlp_train without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same results, bit for bit, as
+ * afx_mlp_train; every step is launched eagerly (no graph is built).  The
+ * minibatch indices, schedule, loss partials and step counter of the call
+ * live in a fixed device array of AFX_MLP_TRAIN_SCRATCH_BYTES per GPU, held
+ * until the process ends, instead of buffers allocated per call.  A call
+ * needs steps * (4 batch + 4 (batch/32)(out_dim/32) + 12) bytes, each of
+ * the three parts rounded up to 256, plus 256; a larger call returns
+ * AFX_E_RANGE (split it).  Calls on one GPU share the array (and a host
+ * staging image of it): calls that overlap on one GPU race, so the library
+ * serialises them within one process. */
+#define AFX_MLP_TRAIN_SCRATCH_BYTES (8u << 20)
+int afx_mlp_train_noalloc(afx_mlp* mlp, int64_t steps, float lr0, float lr1, uint32_t seed, float* loss_out);
+
 int afx_mlp_get_stats(afx_mlp* mlp, afx_mlp_stats* st);
 
 #ifdef __cplusplus
