@@ -20,12 +20,15 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <numeric>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 namespace afx {
 int load_kerkythea(const char* path, float res, int32_t dims[3], std::vector<int32_t>& vox, std::vector<float>& a,
@@ -2126,6 +2129,96 @@ struct TuneChoice {
 std::map<TuneKey, TuneChoice> g_tune_cache;
 std::mutex g_tune_mutex;
 
+// Tuning results also persist across processes in a text file, one line per
+// key:  <device> <prec> <compat> <pml> <flags> <nrooms> <nx ny nz ...> <tile> <tblock> <rotate>.
+// The file is AFX_TUNE_CACHE, else $XDG_CACHE_HOME/afx/tune-v1.txt, else
+// $HOME/.cache/afx/tune-v1.txt; AFX_TUNE_CACHE=off keeps results in the
+// process only.  The device field is the architecture name and CU count, so
+// a file shared between machines keeps their results apart.  Lines that do
+// not parse, or name a tile shape this build lacks, are ignored.
+std::string tune_file_path()
+{
+    const char* e = std::getenv("AFX_TUNE_CACHE");
+    if (e && *e) return std::strcmp(e, "off") == 0 ? std::string() : std::string(e);
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) return std::string(x) + "/afx/tune-v1.txt";
+    if (const char* h = std::getenv("HOME"); h && *h) return std::string(h) + "/.cache/afx/tune-v1.txt";
+    return std::string();
+}
+
+std::string device_tag(int device)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
+    std::string arch(p.gcnArchName);
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    return arch + "x" + std::to_string(p.multiProcessorCount);
+}
+
+std::string tune_line_key(const TuneKey& k, const std::string& dev)
+{
+    std::string out = dev + " " + std::to_string(k.prec) + " " + std::to_string(k.compat) + " " +
+                      std::to_string(k.pml) + " " + std::to_string(k.flags) + " " + std::to_string(k.dims.size() / 3);
+    for (int32_t d : k.dims) out += " " + std::to_string(d);
+    return out;
+}
+
+// Loads the file's entries for this device into g_tune_cache (once per
+// process and device; the caller holds g_tune_mutex).
+void tune_file_load(int device)
+{
+    static std::vector<int> loaded;
+    if (std::find(loaded.begin(), loaded.end(), device) != loaded.end()) return;
+    loaded.push_back(device);
+    const std::string path = tune_file_path();
+    if (path.empty()) return;
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    const std::string dev = device_tag(device);
+    char line[4096];
+    while (std::fgets(line, sizeof line, f)) {
+        char name[128];
+        int prec, compat, pml, nrooms, used = 0;
+        unsigned flags;
+        if (std::sscanf(line, "%127s %d %d %d %u %d%n", name, &prec, &compat, &pml, &flags, &nrooms, &used) != 6)
+            continue;
+        if (dev != name || nrooms < 1 || nrooms > kMaxRooms) continue;
+        TuneKey k;
+        k.prec = prec;
+        k.compat = compat;
+        k.pml = pml;
+        k.flags = flags;
+        k.device = device;
+        const char* q = line + used;
+        bool ok = true;
+        for (int i = 0; i < 3 * nrooms && ok; ++i) {
+            int v, n = 0;
+            ok = std::sscanf(q, "%d%n", &v, &n) == 1;
+            k.dims.push_back(v);
+            q += n;
+        }
+        int tile, tblock, rotate;
+        if (!ok || std::sscanf(q, "%d %d %d", &tile, &tblock, &rotate) != 3) continue;
+        if (tile < 0 || tile >= kNumTiles || (tblock != 1 && tblock != 2)) continue;
+        g_tune_cache[k] = TuneChoice{tile, tblock, rotate != 0};
+    }
+    std::fclose(f);
+}
+
+void tune_file_store(const TuneKey& k, const TuneChoice& c)
+{
+    const std::string path = tune_file_path();
+    if (path.empty()) return;
+    // create the directory chain of the default location
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') (void)mkdir(path.substr(0, i).c_str(), 0755);
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d %d\n", tune_line_key(k, device_tag(k.device)).c_str(), c.tile, c.tblock,
+                 c.rotate ? 1 : 0);
+    std::fclose(f);
+}
+
 }  // namespace
 
 extern "C" int afx_set_tile(afx_sim* s, int32_t tx, int32_t ty, int32_t tz)
@@ -2187,6 +2280,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
     TuneChoice choice{};
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
+        tune_file_load(key.device);
         auto hit = g_tune_cache.find(key);
         if (hit != g_tune_cache.end()) {
             cached = true;
@@ -2251,6 +2345,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         if (int rc = set_tile_tb(s, best.tile, best.tblock, best.rotate)) return rc;
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         g_tune_cache[key] = best;
+        tune_file_store(key, best);
     }
     if (tile) {
         tile[0] = kTiles[s->tile].tx;

@@ -12,6 +12,7 @@
 // compiled with -ffp-contract=off and IEEE division, so FP32 results are
 // bit-identical to the reference operations evaluated in IEEE single.
 #include "afx_internal.hpp"
+#include "afx_launch.hpp"
 
 namespace afx {
 
@@ -946,6 +947,7 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, floa
 
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t s)
 {
+    AFX_CHECK_LAUNCH(dim3(8192), dim3(256), 256);
     hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const float4*)src, (float4*)dst, n16);
     return hipGetLastError();
 }
@@ -1185,6 +1187,7 @@ hipError_t launch_plane_f32(const void* src, Prec prec, float* dst, int nu, int 
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) return hipSuccess;
     const dim3 g((unsigned)blocks), b(256);
+    AFX_CHECK_LAUNCH(g, b, 1024);
     switch (prec) {
     case Prec::F32: hipLaunchKernelGGL(k_plane_f32<float>, g, b, 0, s, (const float*)src, dst, nu, nv, su, sv, base, scale); break;
     case Prec::F64: hipLaunchKernelGGL(k_plane_f32<double>, g, b, 0, s, (const double*)src, dst, nu, nv, su, sv, base, scale); break;
@@ -1199,6 +1202,7 @@ template <typename S, int TX, int TY, int TZ>
 static hipError_t interior_t(const StepArgs& a, hipStream_t s)
 {
     using C = typename Compute<S>::T;
+    AFX_CHECK_LAUNCH(dim3(a.ntiles), dim3(TX * TY), TX * TY);
     hipLaunchKernelGGL((k_interior<S, TX, TY, TZ>), dim3(a.ntiles), dim3(TX * TY), 0, s, (S*)a.out,
                        (const S*)a.cur, (const S*)a.prev, a.tiles, a.rooms, (C)a.l2, (C)a.c2);
     return hipGetLastError();
@@ -1211,6 +1215,7 @@ static hipError_t fused_t(const StepArgs& a, hipStream_t s)
     constexpr int NT = TX * TY;
     BndCtx<S, C> c = make_ctx<S, C>(a);
     if (a.work) {
+        AFX_CHECK_LAUNCH(dim3(a.nwork), dim3(NT), NT);
         hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.colmask, a.d_t,
                            a.t_rel, a.work, a.e_pack, a.pml_nbd);
         return hipGetLastError();
@@ -1219,6 +1224,7 @@ static hipError_t fused_t(const StepArgs& a, hipStream_t s)
     const int nblk_s = (a.nsp + NT - 1) / NT;
     const int nblk_p = (a.npml + NT - 1) / NT;
     const int tile0 = (nblk_b + nblk_s + nblk_p + 7) / 8 * 8;
+    AFX_CHECK_LAUNCH(dim3(tile0 + a.ntiles), dim3(NT), NT);
     hipLaunchKernelGGL((k_fused_blocks<S, TX, TY, TZ>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles,
                        a.colmask, a.nb, nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.d_t, a.t_rel);
     return hipGetLastError();
@@ -1287,6 +1293,7 @@ static hipError_t boundary_s(const StepArgs& a, int skip_special, int special_mo
     const int nblk_s = (nsp + kBndThreads - 1) / kBndThreads;
     if (nblk_b + nblk_s == 0) return hipSuccess;
     BndCtx<S, C> c = make_ctx<S, C>(a);
+    AFX_CHECK_LAUNCH(dim3(nblk_b + nblk_s), dim3(kBndThreads), kBndThreads);
     hipLaunchKernelGGL((k_boundary<S, C>), dim3(nblk_b + nblk_s), dim3(kBndThreads), 0, s, c, a.nb, nblk_b, nsp,
                        a.d_t, a.t_rel, skip_special, special_mode);
     return hipGetLastError();
@@ -1305,6 +1312,7 @@ hipError_t launch_boundary(const StepArgs& a, int skip_special, int special_mode
 
 hipError_t launch_advance_t(int64_t* d_t, int64_t k, hipStream_t s)
 {
+    AFX_CHECK_LAUNCH(dim3(1), dim3(64), 1024);
     hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, s, d_t, k);
     return hipGetLastError();
 }
@@ -1318,6 +1326,7 @@ static dim3 pml_grid(const PmlArgs& a)
 hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return hipSuccess;
+    AFX_CHECK_LAUNCH(pml_grid(a), dim3(256), 1024);
     switch (a.prec) {
     case Prec::F32: hipLaunchKernelGGL(k_pml_p<float>, pml_grid(a), dim3(256), 0, s, a); break;
     case Prec::F64: hipLaunchKernelGGL(k_pml_p<double>, pml_grid(a), dim3(256), 0, s, a); break;
@@ -1330,6 +1339,7 @@ hipError_t launch_pml_phys_p(const PmlArgs& a, hipStream_t s)
 hipError_t launch_pml_phys_phi(const PmlArgs& a, hipStream_t s)
 {
     if (a.nbox <= 0) return hipSuccess;
+    AFX_CHECK_LAUNCH(pml_grid(a), dim3(256), 1024);
     switch (a.prec) {
     case Prec::F32: hipLaunchKernelGGL(k_pml_phi<float>, pml_grid(a), dim3(256), 0, s, a); break;
     case Prec::F64: hipLaunchKernelGGL(k_pml_phi<double>, pml_grid(a), dim3(256), 0, s, a); break;
@@ -1343,6 +1353,7 @@ template <typename S>
 static hipError_t pml_compat_s(const PmlArgs& a, hipStream_t s)
 {
     const dim3 g(2048), b(256);
+    AFX_CHECK_LAUNCH(g, b, 1024);
     hipLaunchKernelGGL((k_pml_compat<S, 0>), g, b, 0, s, a);
     hipLaunchKernelGGL((k_pml_compat<S, 1>), g, b, 0, s, a);
     hipLaunchKernelGGL((k_pml_compat<S, 2>), g, b, 0, s, a);
@@ -1365,6 +1376,7 @@ hipError_t launch_mat_consts(Prec prec, const void* A, const void* B, int N, int
 {
     if (nmat <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)((nmat + 255) / 256);
+    AFX_CHECK_LAUNCH(dim3(blocks), dim3(256), 1024);
     if (prec == Prec::F64)
         hipLaunchKernelGGL(k_mat_consts<double>, dim3(blocks), dim3(256), 0, s, (const double*)A, (const double*)B, N,
                            nmat, lam_b, l2b, (double*)M);
@@ -1380,6 +1392,7 @@ hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s)
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
+    AFX_CHECK_LAUNCH(dim3((unsigned)blocks), dim3(256), 1024);
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)p, n);
     return hipGetLastError();
 }
@@ -1389,6 +1402,7 @@ hipError_t launch_to_f32(const void* src, Prec prec, float* dst, int64_t n, doub
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) return hipSuccess;
+    AFX_CHECK_LAUNCH(dim3((unsigned)blocks), dim3(256), 1024);
     switch (prec) {
     case Prec::F32: hipLaunchKernelGGL(k_to_f32<float>, dim3((unsigned)blocks), dim3(256), 0, s, (const float*)src, dst, n, scale); break;
     case Prec::F64: hipLaunchKernelGGL(k_to_f32<double>, dim3((unsigned)blocks), dim3(256), 0, s, (const double*)src, dst, n, scale); break;

@@ -53,6 +53,7 @@
 
 #include "../../include/afx.h"
 #include "../../include/afx_mlp.h"
+#include "afx_launch.hpp"
 
 namespace afx {
 int report_error(int code, const char* msg);
@@ -720,6 +721,7 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
     if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 128 == 0 && g.N % 128 == 0 &&
         g.K % 256 == 0) {
         const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(512);
+        HIP_CHECK(afx::launch_ok(grid128, block128, 512));
         if (epi == EPI_HIDDEN)
             k_gemm128<EPI_HIDDEN><<<grid128, block128, 0, s->stream>>>(g);
         else
@@ -730,6 +732,7 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
     if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 64 == 0 && g.N % 64 == 0 &&
         g.K % 128 == 0) {
         const dim3 grid64((unsigned)((g.M / 64) * (g.N / 64))), block64(256);
+        HIP_CHECK(afx::launch_ok(grid64, block64, 256));
         if (epi == EPI_HIDDEN)
             k_gemm64<EPI_HIDDEN><<<grid64, block64, 0, s->stream>>>(g);
         else
@@ -738,6 +741,7 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
         return AFX_OK;
     }
     const dim3 grid((unsigned)((g.M / 32) * (g.N / 32))), block(256);
+    HIP_CHECK(afx::launch_ok(grid, block, 256));
     switch (epi) {
     case EPI_HIDDEN: k_gemm<EPI_HIDDEN><<<grid, block, 0, s->stream>>>(g); break;
     case EPI_OUT: k_gemm<EPI_OUT><<<grid, block, 0, s->stream>>>(g); break;
@@ -759,6 +763,7 @@ hipError_t launch_features(afx_mlp* s, const float* x, const int* rows, int nrow
     const int per_row = s->cfg.n_freq * s->cfg.in_dim;
     const int rz = std::max(1, std::min(256 / per_row, 1024 / per_row));
     const dim3 block((unsigned)s->cfg.n_freq, (unsigned)s->cfg.in_dim, (unsigned)rz);
+    if (hipError_t le = afx::launch_ok(dim3((unsigned)((nrows + rz - 1) / rz)), block, 1024)) return le;
     k_features<<<dim3((unsigned)((nrows + rz - 1) / rz)), block, 0, s->stream>>>(x, rows, nrows, s->cfg.in_dim,
                                                                                s->cfg.n_freq, s->freq, X, XT, ldt, sr);
     return hipGetLastError();
@@ -873,7 +878,9 @@ int step_grad(afx_mlp* s, const float* x, const float* target, const float* weig
         ct.rows[l] = s->out[l];
         ct.start[l + 1] = ct.start[l] + s->out[l];
     }
-    k_grads<<<gt.start[L] + (ct.start[L] + 3) / 4, 256, 0, s->stream>>>(gt, ct, ad, s->sr);
+    const dim3 ggrid((unsigned)(gt.start[L] + (ct.start[L] + 3) / 4));
+    HIP_CHECK(afx::launch_ok(ggrid, dim3(256), 256));
+    k_grads<<<ggrid, 256, 0, s->stream>>>(gt, ct, ad, s->sr);
     HIP_CHECK(hipGetLastError());
     return AFX_OK;
 }
@@ -900,6 +907,7 @@ int adam(afx_mlp* s, float lr, bool update)
         bc2 = (float)(1.0 - std::pow((double)s->cfg.beta2, (double)s->t));
     }
     const int64_t blocks = at.tstart[s->L] + (at.bofs[s->L] + 255) / 256;
+    HIP_CHECK(afx::launch_ok(dim3((unsigned)blocks), dim3(256), 256));
     k_adam<<<(unsigned)blocks, 256, 0, s->stream>>>(at, s->w, s->m1, s->m2, s->g, lr, s->cfg.beta1, s->cfg.beta2,
                                                     s->cfg.eps, s->cfg.weight_decay, bc1, bc2, update ? 1 : 0, s->sr);
     HIP_CHECK(hipGetLastError());

@@ -34,12 +34,30 @@ def progress(msg):
 BYTES_PER_CELL = {"fp32": 12, "fp64": 24, "fp16": 6, "bf16": 6}
 
 
+def host_cores():
+    """(threads to use, CPUs visible): every CPU this process may run on
+    (its affinity set), limited to the CPU share the host grants it (the
+    cgroup cpu.max quota, and OMP_NUM_THREADS when the host sets it)."""
+    visible = len(os.sched_getaffinity(0))
+    share = visible
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            share = min(share, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return share, visible
+
+
 def cpu_baseline(w, budget_s=12.0):
     """Time the CPU oracle (OpenMP C restatement) on the same room for a
-    bounded number of steps on this host's cores."""
+    bounded number of steps on every host core granted to this process."""
     import numpy as np
     import orc
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores, visible = host_cores()
     o = orc.from_workload(w, compat=False, threads=cores)
     o.run(5)  # first touch
     steps, t0 = 0, time.perf_counter()
@@ -49,7 +67,7 @@ def cpu_baseline(w, budget_s=12.0):
     dt = time.perf_counter() - t0
     o.close()
     return {"value": round(w.cells * steps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
-            "kind": "port",
+            "cpus_visible": visible, "kind": "port",
             "sample": f"{w.name} {w.dims[0]}x{w.dims[1]}x{w.dims[2]}, {steps} steps after 5 warm-up steps, "
                       f"FP32 oracle (C, OpenMP) on {cores} host threads"}
 
@@ -87,7 +105,7 @@ def configs_leg():
                "fp32_hbm_roofline_frac_algorithmic": round(gps * 12 / HBM_PEAK_GBS, 4),
                "rel_l2_fp32_vs_fp64": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
         if w.cells < 10 ** 6:
-            cores = min(len(os.sched_getaffinity(0)), 16)
+            cores = host_cores()[0]
             o = orc.from_workload(w, compat=False, threads=cores)
             t0 = time.perf_counter()
             o.run(w.steps)

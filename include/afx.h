@@ -206,7 +206,11 @@ int afx_get_time_block(afx_sim* sim, int32_t* steps, int64_t* pair_tiles);
  * current setting on a copy of the state; the best of three timed passes
  * counts), keeps the fastest and returns its tile in tile[3]
  * (afx_get_time_block gives its steps per pass).  Leaves the simulation
- * state unchanged. */
+ * state unchanged.  Results are kept per (device, precision, mode, PML,
+ * flags, room dims) for the process and in the file AFX_TUNE_CACHE (default
+ * $XDG_CACHE_HOME/afx/tune-v1.txt or $HOME/.cache/afx/tune-v1.txt;
+ * "off" disables it), so a later process with the same key skips the
+ * search. */
 int afx_autotune(afx_sim* sim, int32_t steps_per_trial, int32_t tile[3]);
 
 /* Measurement aid, no counterpart in the reference: a plain device copy of
