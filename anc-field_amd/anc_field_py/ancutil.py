@@ -52,9 +52,16 @@ def EstimateIr(x, y, n, mu=1e-4, epochs=16):
     return w
 
 
-def impulse_response(trace, fs, cutoff=2000.0, decimate=8, ntaps=255):
-    """Band-limit a receiver trace at `cutoff` Hz with a zero-phase FIR and
-    keep every `decimate`-th sample."""
+def impulse_response(trace, fs, cutoff=2000.0, decimate=4, ntaps=255):
+    """The secondary path at fs/decimate from the receiver trace of a unit
+    impulse: band-limited at `cutoff` Hz with a zero-phase FIR, every
+    `decimate`-th sample kept, times `decimate`.
+
+    The factor makes it the filter the reference's identification estimates:
+    EstimateIr fitted to the input and output decimated by 4
+    (anc_field_py/identification.py:36) is the decimated-rate system, whose
+    taps are `decimate` times the decimated impulse response of the full-rate
+    one, for band-limited inputs."""
     h = signal.firwin(ntaps, cutoff / (fs / 2.0))
     y = signal.filtfilt(h, [1.0], np.asarray(trace, np.float64), axis=-1)
-    return y[..., ::decimate].astype(np.float32)
+    return (decimate * y[..., ::decimate]).astype(np.float32)
