@@ -591,7 +591,7 @@ int build_special(afx_sim* s)
         std::vector<int32_t> los;
         build_tiles(s, table, los);
         const TileShape ts = kTiles[s->tile];
-        const int NT = ts.tx * ts.ty;
+        const int NT = ts.tx / ts.vec * ts.ty;  // threads per workgroup: one list entry each
         const size_t nslot = table.size();
         const TileRanker ranker(s);
         const int64_t per = ranker.per;
@@ -634,7 +634,7 @@ int build_special(afx_sim* s)
         // steps per pass are selected.
         std::vector<uint8_t> pair_ok(nslot, 0);
         int64_t npair = 0;
-        if (s->tblock == 2 && ts.ty >= 2)
+        if (s->tblock == 2 && ts.ty >= 2 && ts.vec == 1)
             for (size_t sl = 0; sl < nslot; ++sl) {
                 const int4 t4 = table[sl];
                 if (t4.x < 0) continue;
@@ -793,7 +793,7 @@ int build_special(afx_sim* s)
         std::vector<int32_t> los;
         build_tiles(s, table, los);
         const TileShape ts = kTiles[s->tile];
-        const int NT = ts.tx * ts.ty;
+        const int NT = ts.tx * ts.ty;  // columns per tile slot
         std::vector<uint32_t> col(std::max<size_t>(table.size() * (size_t)NT, 1), 0u);
         for (size_t sl = 0; sl < table.size(); ++sl) {
             const int4 t4 = table[sl];
@@ -978,6 +978,8 @@ static void step_scalars(const afx_sim* s, StepArgs& a)
     }
 }
 
+bool tile_fits(const afx_sim* s, int tile);
+
 int commit(afx_sim* s)
 {
     if (s->committed) {
@@ -1006,12 +1008,17 @@ int commit(afx_sim* s)
     if (s->has_model && s->ncoef <= 0) return fail(AFX_E_STATE, "voxels set but no materials");
     if (s->ncoef > kMaxCoef) return fail(AFX_E_ARG, "ncoef %d above the supported %d", s->ncoef, kMaxCoef);
     if ((int)s->rooms.size() > kMaxRooms) return fail(AFX_E_ARG, "more than %d rooms", kMaxRooms);
+    // rooms start on 64-cell boundaries, so tiles with vector lanes read
+    // aligned vectors in every room
     int64_t base = 0;
     for (auto& r : s->rooms) {
         r.base = base;
-        base += (int64_t)r.nx * r.ny * r.nz;
+        base += ((int64_t)r.nx * r.ny * r.nz + 63) / 64 * 64;
     }
     s->total_cells = base;
+    if (!tile_fits(s, s->tile))
+        return fail(AFX_E_ARG, "tile %dx%dx%d needs room widths that are multiples of %d", kTiles[s->tile].tx,
+                    kTiles[s->tile].ty, kTiles[s->tile].tz, kTiles[s->tile].vec);
     if (base >= (int64_t)INT32_MAX - 1) return fail(AFX_E_RANGE, "%lld cells exceed the 2^31 index range", (long long)base);
     if (s->pml_mode == 1 && s->rooms.size() > 1) return fail(AFX_E_ARG, "compat PML supports a single room");
     if (s->pml_mode == 2)
@@ -1972,7 +1979,8 @@ int afx_get_stats(afx_sim* s, afx_stats* st)
     if (int rc = check_sim(s)) return rc;
     if (!st) return fail(AFX_E_ARG, "null stats");
     std::memset(st, 0, sizeof *st);
-    st->cells = s->total_cells;
+    st->cells = 0;
+    for (const Room& r : s->rooms) st->cells += (int64_t)r.nx * r.ny * r.nz;
     st->boundary = s->nb;
     for (int g = 0; g < 26; ++g) {
         st->group_count[g] = s->group_count[g];
@@ -2105,7 +2113,19 @@ int set_tile(afx_sim* s, int tile) { return set_tile_tb(s, tile, s->tblock, s->x
 
 bool can_pair(const afx_sim* s, int tile)
 {
-    return use_fused(s) && !(s->cfg.flags & AFX_LIST_BLOCKS) && kTiles[tile].ty >= 2;
+    return use_fused(s) && !(s->cfg.flags & AFX_LIST_BLOCKS) && kTiles[tile].ty >= 2 && kTiles[tile].vec == 1;
+}
+
+// A tile shape with V cells per lane needs every room's width to be a
+// multiple of V (room bases are aligned at commit), so each lane's cells are
+// one aligned vector.
+bool tile_fits(const afx_sim* s, int tile)
+{
+    const int v = kTiles[tile].vec;
+    if (v <= 1) return true;
+    for (const Room& r : s->rooms)
+        if (r.nx % v != 0) return false;
+    return true;
 }
 
 struct TuneKey {
@@ -2230,6 +2250,9 @@ extern "C" int afx_set_tile(afx_sim* s, int32_t tx, int32_t ty, int32_t tz)
                 s->tile = k;
                 return AFX_OK;
             }
+            if (!tile_fits(s, k))
+                return fail(AFX_E_ARG, "tile %dx%dx%d needs room widths that are multiples of %d", tx, ty, tz,
+                            kTiles[k].vec);
             HIP_TRY(hipSetDevice(s->cfg.device));
             HIP_TRY(hipStreamSynchronize(s->stream));
             return set_tile(s, k);
@@ -2300,7 +2323,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         // bit-exactness check
         std::vector<TuneChoice> cands = {current};
         for (int tl = 0; tl < kNumTiles; ++tl)
-            for (int tb = 1; tb <= (can_pair(s, tl) ? 2 : 1); ++tb)
+            for (int tb = 1; tb <= (can_pair(s, tl) ? 2 : 1) && tile_fits(s, tl); ++tb)
                 for (int ro = 1; ro >= 0; --ro) {
                     const TuneChoice c{tl, tb, ro != 0};
                     if (c != current) cands.push_back(c);

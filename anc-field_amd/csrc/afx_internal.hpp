@@ -35,11 +35,14 @@ constexpr int kMaxRooms = 256;
 constexpr int kMaxMaterials = 1024;  // 10-bit material ids, three per entry word
 constexpr int kMaxCoef = 8;
 
-// Interior tile shapes the step kernels are instantiated for.
+// Interior tile shapes the step kernels are instantiated for: tx x ty x tz
+// cells.  vec > 1: each lane owns vec consecutive cells of x (one 64-lane
+// wave per row, tx = 64 vec), read and written as one vector; such tiles
+// need every room's nx to be a multiple of vec.
 struct TileShape {
-    int tx, ty, tz;
+    int tx, ty, tz, vec;
 };
-constexpr int kNumTiles = 7;
+constexpr int kNumTiles = 11;
 extern const TileShape kTiles[kNumTiles];
 
 enum class Prec : int { F32 = 0, F64 = 1, F16 = 2, BF16 = 3 };

@@ -17,7 +17,8 @@
 namespace afx {
 
 const TileShape kTiles[kNumTiles] = {
-    {64, 4, 16}, {64, 4, 32}, {64, 8, 16}, {64, 2, 32}, {128, 2, 16}, {64, 4, 8}, {256, 1, 16},
+    {64, 4, 16, 1}, {64, 4, 32, 1}, {64, 8, 16, 1}, {64, 2, 32, 1}, {128, 2, 16, 1}, {64, 4, 8, 1}, {256, 1, 16, 1},
+    {256, 4, 8, 4}, {256, 2, 8, 4}, {128, 4, 16, 2}, {128, 4, 8, 2},
 };
 
 // ---------------------------------------------------------------- storage I/O
@@ -173,6 +174,146 @@ __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __
     }
 }
 
+// V consecutive cells of x per lane, one 64-lane wave per row of the tile:
+// the lane's V cells of each plane are read and written as one vector, and
+// the x neighbours across lanes come from the adjacent lanes' registers
+// (lane 0 and lane 63 read their outer neighbour).  Otherwise as
+// interior_column: z-march with p^n at z-1, z, z+1 in registers, planes
+// prefetched two ahead, the same operations in the same order per cell, and
+// cells whose skip bit is set left alone.  Needs x, nx and the room base to
+// be multiples of V (aligned vectors); lanes past the row's end read the
+// row's last vector and store nothing.
+template <typename S, int V> struct alignas(sizeof(S) * V) VecPack {
+    S v[V];
+};
+
+template <typename S, typename C, int V>
+__device__ __forceinline__ void ldv(C (&o)[V], const S* __restrict__ p, int64_t i)
+{
+    const VecPack<S, V> t = *reinterpret_cast<const VecPack<S, V>*>(p + i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) o[k] = static_cast<C>(t.v[k]);
+}
+
+template <typename S, typename C, int V, int TZ>
+__device__ __forceinline__ void interior_vec(S* __restrict__ out, const S* __restrict__ cur,
+                                             const S* __restrict__ prev, const RoomDev& r, int x, int y, int z0,
+                                             C l2, C c2, const uint32_t (&skip)[V])
+{
+    if (y < 1 || y >= r.ny - 1) return;  // the whole wave: one row
+    const int zs = z0 < 1 ? 1 : z0;
+    const int ze = (z0 + TZ) < (r.nz - 1) ? (z0 + TZ) : (r.nz - 1);
+    if (zs >= ze) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    const bool inrow = x < r.nx;
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + (inrow ? x : r.nx - V);
+    const int xe = lane == 0 ? (x >= 1 ? x - 1 : 0) : (x + V < r.nx ? x + V : r.nx - 1);
+    const int64_t ie = r.base + (int64_t)zs * sz + (int64_t)y * sy + xe;
+    bool can[V];  // cell inside the row's updated range [1, nx-2]
+    bool all = inrow;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        can[k] = inrow && x + k >= 1 && x + k <= r.nx - 2;
+        all = all && can[k];
+    }
+    constexpr int kDepth = 2;
+    struct Plane {
+        C u[V], yp[V], ym[V], pm[V];  // p^n at z+1, y+1, y-1; p^{n-1}
+        C e;                          // p^n of the lane's outer x neighbour at z+1
+    };
+    auto load_plane = [&](Plane& P, int64_t j, int64_t je) {
+        ldv<S, C, V>(P.u, cur, j + sz);
+        ldv<S, C, V>(P.yp, cur, j + sy);
+        ldv<S, C, V>(P.ym, cur, j - sy);
+        ldv<S, C, V>(P.pm, prev, j);
+        P.e = ld<S, C>(cur, je + sz);
+    };
+    const int64_t i_last = i + (int64_t)(ze - 1 - zs) * sz, ie_last = ie + (int64_t)(ze - 1 - zs) * sz;
+    C pd[V], pc[V];
+    ldv<S, C, V>(pd, cur, i - sz);
+    ldv<S, C, V>(pc, cur, i);
+    C ec = ld<S, C>(cur, ie);
+    Plane P[kDepth];
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d)
+        load_plane(P[d], zs + d < ze ? i + (int64_t)d * sz : i_last, zs + d < ze ? ie + (int64_t)d * sz : ie_last);
+    for (int z = zs; z < ze; z += kDepth) {
+#pragma unroll
+        for (int d = 0; d < kDepth; ++d) {
+            const int64_t j = i + (int64_t)d * sz;
+            if (z + d < ze) {  // uniform across the wave
+                const C left_in = __shfl_up(pc[V - 1], 1);
+                const C right_in = __shfl_down(pc[0], 1);
+                const C left = lane == 0 ? ec : left_in;
+                const C right = lane == 63 ? ec : right_in;
+                VecPack<S, V> o;
+                bool full = all;
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const C xp = k == V - 1 ? right : pc[k + 1];
+                    const C xm = k == 0 ? left : pc[k - 1];
+                    C s = xp + xm;
+                    s = s + P[d].yp[k];
+                    s = s + P[d].ym[k];
+                    s = s + P[d].u[k];
+                    s = s + pd[k];
+                    const C v = l2 * s + c2 * pc[k];
+                    o.v[k] = static_cast<S>(v - P[d].pm[k]);
+                    full = full && !((skip[k] >> (z + d - z0)) & 1u);
+                }
+                if (full) {
+                    *reinterpret_cast<VecPack<S, V>*>(out + j) = o;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < V; ++k)
+                        if (can[k] && !((skip[k] >> (z + d - z0)) & 1u)) out[j + k] = o.v[k];
+                }
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    pd[k] = pc[k];
+                    pc[k] = P[d].u[k];
+                }
+                ec = P[d].e;
+            }
+            const bool more = z + d + kDepth < ze;
+            load_plane(P[d], more ? j + (int64_t)kDepth * sz : i_last,
+                       more ? ie + (int64_t)(z + d + kDepth - zs) * sz : ie_last);
+        }
+        i += (int64_t)kDepth * sz;
+    }
+}
+
+// The interior cells of one tile (origin ox, oy, oz of room r), one column
+// per thread or V columns per lane; cmask: the tile slot's column masks
+// (TX x TY words, bit z - oz set for cells left to a list), or null.
+template <typename S, typename C, int TX, int TY, int TZ, int V>
+__device__ __forceinline__ void tile_interior(S* __restrict__ out, const S* __restrict__ cur,
+                                              const S* __restrict__ prev, const RoomDev& r, int ox, int oy, int oz,
+                                              C l2, C c2, const uint32_t* __restrict__ cmask)
+{
+    if constexpr (V == 1) {
+        const int x = ox + (int)(threadIdx.x % TX);
+        const int y = oy + (int)(threadIdx.x / TX);
+        const uint32_t bits = cmask ? cmask[threadIdx.x] : 0u;
+        interior_column<S, C, TZ>(out, cur, prev, r, x, y, oz, l2, c2, bits);
+    } else {
+        static_assert(TX == 64 * V, "one 64-lane wave per tile row");
+        const int lane = (int)(threadIdx.x & 63u), row = (int)(threadIdx.x >> 6);
+        uint32_t bits[V];
+        if (cmask) {
+            const VecPack<uint32_t, V> w =
+                *reinterpret_cast<const VecPack<uint32_t, V>*>(cmask + row * TX + lane * V);
+#pragma unroll
+            for (int k = 0; k < V; ++k) bits[k] = w.v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < V; ++k) bits[k] = 0u;
+        }
+        interior_vec<S, C, V, TZ>(out, cur, prev, r, ox + lane * V, oy + row, oz, l2, c2, bits);
+    }
+}
+
 // Two steps of one tile whose cells and one-cell halo are all interior cells
 // (none is a boundary, PML or source/receiver cell or lies on the grid edge).
 // Step n+1 runs on the tile and its halo, z-plane by z-plane, from planes of
@@ -290,19 +431,17 @@ __device__ __forceinline__ void interior_pair(S* __restrict__ o1, S* __restrict_
     }
 }
 
-template <typename S, int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX * TY) void k_interior(S* __restrict__ out, const S* __restrict__ cur,
-                                                      const S* __restrict__ prev, const int4* __restrict__ tiles,
-                                                      const RoomDev* __restrict__ rooms,
-                                                      typename Compute<S>::T l2, typename Compute<S>::T c2)
+template <typename S, int TX, int TY, int TZ, int V>
+__global__ __launch_bounds__(TX / V * TY) void k_interior(S* __restrict__ out, const S* __restrict__ cur,
+                                                          const S* __restrict__ prev, const int4* __restrict__ tiles,
+                                                          const RoomDev* __restrict__ rooms,
+                                                          typename Compute<S>::T l2, typename Compute<S>::T c2)
 {
     using C = typename Compute<S>::T;
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const RoomDev r = rooms[t.x];
-    const int x = t.y + (int)(threadIdx.x % TX);
-    const int y = t.z + (int)(threadIdx.x / TX);
-    interior_column<S, C, TZ>(out, cur, prev, r, x, y, t.w, l2, c2, 0u);
+    tile_interior<S, C, TX, TY, TZ, V>(out, cur, prev, r, t.y, t.z, t.w, l2, c2, nullptr);
 }
 
 // ---------------------------------------------------------------- boundary
@@ -841,8 +980,8 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary(BndCtx<S, C> c, int nb
 //   [tile0, tile0+ntiles)  interior tiles; tile0 is a multiple of 8 so the
 //                          XCD-swizzled tile order keeps its XCD mapping
 // The latency-bound lists come first so they overlap the interior stream.
-template <typename S, int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>::T> c,
+template <typename S, int TX, int TY, int TZ, int V>
+__global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Compute<S>::T> c,
                                                    const int4* __restrict__ tiles,
                                                    const uint32_t* __restrict__ colmask,
                                                    const int64_t* __restrict__ d_t, int t_rel,
@@ -851,8 +990,10 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
                                                    const uint32_t* __restrict__ pml_nbd)
 {
     using C = typename Compute<S>::T;
-    constexpr int NT = TX * TY;
-    constexpr int kLds = PairLds<TX, TY>::size > 3 * NT ? PairLds<TX, TY>::size : 3 * NT;
+    constexpr int NT = TX / V * TY;   // threads
+    constexpr int NCOL = TX * TY;     // columns (column-mask words per tile slot)
+    constexpr int kPair = V == 1 ? PairLds<TX, TY>::size : 0;
+    constexpr int kLds = kPair > 3 * NT ? kPair : 3 * NT;
     __shared__ C lds[kLds];  // PML chunks: phi^n by thread; two-step tiles: pressure planes
     // Work table: item (kind, a, b, w).  kind 0: interior tile of slot a,
     // origin packed in b (x | y << 11 | z << 22), room w; 4: the same tile
@@ -862,7 +1003,9 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
     // chunk right before the tile holding its first cell, so both read the
     // same planes of p^n through L2.
     const int4 wk = work[blockIdx.x];
-    if constexpr (PairLds<TX, TY>::size > 0) {
+    if constexpr (kPair == 0) {
+        if (wk.x == 4) return;  // no two-step tiles are dealt to these shapes
+    } else {
         if (wk.x == 4) {
             const RoomDev r = wk.w == 0 ? c.room0 : c.rooms[wk.w];
             const uint32_t o = (uint32_t)wk.z;
@@ -874,10 +1017,8 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
     if (wk.x == 0) {
         const RoomDev r = wk.w == 0 ? c.room0 : c.rooms[wk.w];
         const uint32_t o = (uint32_t)wk.z;
-        const int x = (int)(o & 2047u) + (int)(threadIdx.x % TX);
-        const int y = (int)((o >> 11) & 2047u) + (int)(threadIdx.x / TX);
-        const uint32_t bits = colmask[(int64_t)wk.y * NT + threadIdx.x];
-        interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, (int)(o >> 22), c.k.l2, c.k.c2, bits);
+        tile_interior<S, C, TX, TY, TZ, V>(c.out, c.cur, c.prev, r, (int)(o & 2047u), (int)((o >> 11) & 2047u),
+                                           (int)(o >> 22), c.k.l2, c.k.c2, colmask + (int64_t)wk.y * NCOL);
         return;
     }
     if (wk.x < 0) return;
@@ -902,25 +1043,24 @@ __global__ __launch_bounds__(TX * TY) void k_fused(BndCtx<S, typename Compute<S>
 //   [.., +nblk_p)          PML cells (phi, psi and pressure)
 //   [tile0, tile0+ntiles)  interior tiles; tile0 is a multiple of 8 so the
 //                          XCD-swizzled tile order keeps its XCD mapping
-template <typename S, int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX * TY) void k_fused_blocks(BndCtx<S, typename Compute<S>::T> c,
+template <typename S, int TX, int TY, int TZ, int V>
+__global__ __launch_bounds__(TX / V * TY) void k_fused_blocks(BndCtx<S, typename Compute<S>::T> c,
                                                           const int4* __restrict__ tiles,
                                                           const uint32_t* __restrict__ colmask, int nb, int nblk_b,
                                                           int nsp, int nblk_s, int npml, int nblk_p, int tile0,
                                                           const int64_t* __restrict__ d_t, int t_rel)
 {
     using C = typename Compute<S>::T;
-    constexpr int NT = TX * TY;
+    constexpr int NT = TX / V * TY;   // threads
+    constexpr int NCOL = TX * TY;     // columns
     const int bid = (int)blockIdx.x;
     if (bid >= tile0) {
         const int slot = bid - tile0;
         const int4 t4 = tiles[slot];
         if (t4.x < 0) return;
         const RoomDev r = c.rooms[t4.x];
-        const int x = t4.y + (int)(threadIdx.x % TX);
-        const int y = t4.z + (int)(threadIdx.x / TX);
-        const uint32_t bits = colmask[(int64_t)slot * NT + threadIdx.x];
-        interior_column<S, C, TZ>(c.out, c.cur, c.prev, r, x, y, t4.w, c.k.l2, c.k.c2, bits);
+        tile_interior<S, C, TX, TY, TZ, V>(c.out, c.cur, c.prev, r, t4.y, t4.z, t4.w, c.k.l2, c.k.c2,
+                                           colmask + (int64_t)slot * NCOL);
         return;
     }
     const int64_t t = *d_t + t_rel;
@@ -1198,25 +1338,26 @@ hipError_t launch_plane_f32(const void* src, Prec prec, float* dst, int nu, int 
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename S, int TX, int TY, int TZ>
+template <typename S, int TX, int TY, int TZ, int V = 1>
 static hipError_t interior_t(const StepArgs& a, hipStream_t s)
 {
     using C = typename Compute<S>::T;
-    AFX_CHECK_LAUNCH(dim3(a.ntiles), dim3(TX * TY), TX * TY);
-    hipLaunchKernelGGL((k_interior<S, TX, TY, TZ>), dim3(a.ntiles), dim3(TX * TY), 0, s, (S*)a.out,
+    constexpr int NT = TX / V * TY;
+    AFX_CHECK_LAUNCH(dim3(a.ntiles), dim3(NT), NT);
+    hipLaunchKernelGGL((k_interior<S, TX, TY, TZ, V>), dim3(a.ntiles), dim3(NT), 0, s, (S*)a.out,
                        (const S*)a.cur, (const S*)a.prev, a.tiles, a.rooms, (C)a.l2, (C)a.c2);
     return hipGetLastError();
 }
 
-template <typename S, int TX, int TY, int TZ>
+template <typename S, int TX, int TY, int TZ, int V = 1>
 static hipError_t fused_t(const StepArgs& a, hipStream_t s)
 {
     using C = typename Compute<S>::T;
-    constexpr int NT = TX * TY;
+    constexpr int NT = TX / V * TY;
     BndCtx<S, C> c = make_ctx<S, C>(a);
     if (a.work) {
         AFX_CHECK_LAUNCH(dim3(a.nwork), dim3(NT), NT);
-        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.colmask, a.d_t,
+        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ, V>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.colmask, a.d_t,
                            a.t_rel, a.work, a.e_pack, a.pml_nbd);
         return hipGetLastError();
     }
@@ -1225,7 +1366,7 @@ static hipError_t fused_t(const StepArgs& a, hipStream_t s)
     const int nblk_p = (a.npml + NT - 1) / NT;
     const int tile0 = (nblk_b + nblk_s + nblk_p + 7) / 8 * 8;
     AFX_CHECK_LAUNCH(dim3(tile0 + a.ntiles), dim3(NT), NT);
-    hipLaunchKernelGGL((k_fused_blocks<S, TX, TY, TZ>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles,
+    hipLaunchKernelGGL((k_fused_blocks<S, TX, TY, TZ, V>), dim3(tile0 + a.ntiles), dim3(NT), 0, s, c, a.tiles,
                        a.colmask, a.nb, nblk_b, a.nsp, nblk_s, a.npml, nblk_p, tile0, a.d_t, a.t_rel);
     return hipGetLastError();
 }
@@ -1241,6 +1382,10 @@ static hipError_t interior_s(const StepArgs& a, hipStream_t s)
     case 4: return interior_t<S, 128, 2, 16>(a, s);
     case 5: return interior_t<S, 64, 4, 8>(a, s);
     case 6: return interior_t<S, 256, 1, 16>(a, s);
+    case 7: return interior_t<S, 256, 4, 8, 4>(a, s);
+    case 8: return interior_t<S, 256, 2, 8, 4>(a, s);
+    case 9: return interior_t<S, 128, 4, 16, 2>(a, s);
+    case 10: return interior_t<S, 128, 4, 8, 2>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1256,6 +1401,10 @@ static hipError_t fused_s(const StepArgs& a, hipStream_t s)
     case 4: return fused_t<S, 128, 2, 16>(a, s);
     case 5: return fused_t<S, 64, 4, 8>(a, s);
     case 6: return fused_t<S, 256, 1, 16>(a, s);
+    case 7: return fused_t<S, 256, 4, 8, 4>(a, s);
+    case 8: return fused_t<S, 256, 2, 8, 4>(a, s);
+    case 9: return fused_t<S, 128, 4, 16, 2>(a, s);
+    case 10: return fused_t<S, 128, 4, 8, 2>(a, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -424,6 +424,8 @@ def test_large_room_linearity():
 
 
 TILES = [(64, 4, 16), (64, 4, 32), (64, 8, 16), (64, 2, 32), (128, 2, 16), (64, 4, 8), (256, 1, 16)]
+# shapes with 4 or 2 cells per lane (room widths must be multiples of that)
+VEC_TILES = [(256, 4, 8), (256, 2, 8), (128, 4, 16), (128, 4, 8)]
 
 
 def _pml_room():
@@ -494,7 +496,7 @@ def test_tile_change_mid_run_and_autotune_keep_state():
     s.set_tile(128, 2, 16)
     s.run(50)
     tile = s.autotune(10)
-    assert tile in TILES
+    assert tile in TILES + VEC_TILES
     s.run(60)
     got = s.receivers()
     s.close()
