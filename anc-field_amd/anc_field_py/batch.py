@@ -47,6 +47,55 @@ def run_batched(workloads, steps, precision="fp32", tune=False, **kw):
     return tr, time.perf_counter() - t0, tile
 
 
+STORAGE_BYTES = {"fp32": 4, "fp64": 8, "fp16": 2, "bf16": 2}
+
+
+def cache_groups(workloads, precision="fp32", budget=192 << 20, fields=4):
+    """Rooms split into consecutive groups whose pressure fields (`fields`
+    of them, per cell) fit `budget` bytes, so a group's whole working set
+    stays in the 256 MB Infinity Cache while it is stepped.  Returns lists of
+    indices into workloads; a room larger than the budget is a group alone."""
+    per_cell = fields * STORAGE_BYTES[precision]
+    groups, cur, used = [], [], 0
+    for i, w in enumerate(workloads):
+        b = w.cells * per_cell
+        if cur and used + b > budget:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += b
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def run_grouped(workloads, steps, precision="fp32", tune=False, budget=192 << 20, **kw):
+    """The batch stepped group by group (cache_groups): each group is one
+    handle run for all `steps` before the next starts.  Returns (traces in
+    workload order, wall seconds excluding tuning, groups)."""
+    groups = cache_groups(workloads, precision, budget)
+    out = [None] * len(workloads)
+    wall = 0.0
+    for g in groups:
+        ws = [workloads[i] for i in g]
+        t0 = time.perf_counter()
+        s = make_batch(ws, precision, **kw)
+        if tune:
+            t1 = time.perf_counter()
+            s.autotune(20)
+            t0 += time.perf_counter() - t1
+        s.run(steps)
+        tr = s.receivers()
+        s.close()
+        wall += time.perf_counter() - t0
+        k = 0
+        for i, w in zip(g, ws):
+            n = len(w.mic_cells)
+            out[i] = tr[k:k + n]
+            k += n
+    return np.concatenate(out, axis=0), wall, groups
+
+
 def run_sequential(workloads, steps, precision="fp32", **kw):
     """One handle per room, run back to back."""
     from .rooms import make_sim

@@ -150,6 +150,16 @@ def batched_leg(steps):
             "gcell_per_s": round(cells * steps / t / 1e9, 1), "autotuned_tile": list(tile[:3]), "steps_per_pass": int(tile[3]),
             "rel_l2_vs_fp64": batch.rel_l2(tr, ref)}
     res["batched_fp32"]["bitexact_vs_sequential"] = bool((runs["fp32"][0] == seq).all())
+    # the same batch stepped in groups whose fields fit the Infinity Cache
+    for prec in ("fp32", "fp16"):
+        tr, t, groups = batch.run_grouped(ws, steps, prec, tune=True)
+        res[f"batched_cache_groups_{prec}"] = {
+            "ir_per_s": round(n_ir / t, 2), "wall_s": round(t, 3), "speedup_vs_sequential_fp32": round(t_seq / t, 2),
+            "gcell_per_s": round(cells * steps / t / 1e9, 1), "groups": len(groups),
+            "rel_l2_vs_fp64": batch.rel_l2(tr, ref)}
+        if prec == "fp32":
+            res[f"batched_cache_groups_{prec}"]["bitexact_vs_sequential"] = bool((tr == seq).all())
+        progress(f"batched: cache groups {prec} done")
     return res
 
 
@@ -308,8 +318,13 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
         base = prec.split("_")[0]
         s = rooms.make_sim(w, _afx.Sim, precision=base)
         try:
+            tuned = None
             if two:
                 s.set_time_block(2)
+            elif base in ("fp16", "bf16", "fp32"):
+                t0 = time.perf_counter()
+                tuned = list(s.autotune(8)) + [s.time_block()[0]]
+                tune_s = time.perf_counter() - t0
             s.run(1)
             s.synchronize()
             t0 = time.perf_counter()
@@ -321,6 +336,9 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
         finally:
             s.close()
         res[prec] = {"gcell_per_s": round(w.cells * (steps - 1) / dt / 1e9, 1), "wall_s": round(dt, 2)}
+        if tuned is not None:
+            res[prec]["autotuned_tile_and_steps_per_pass"] = tuned
+            res[prec]["tune_s"] = round(tune_s, 1)
         progress(f"large: {prec} done")
         if two:
             res[prec]["pair_tiles"] = int(pairs)

@@ -68,6 +68,10 @@ def test_batch_48_rooms_full_size(batch48):
     _case("48 rooms one handle each")
     seq, _ = batch.run_sequential(ws, steps)
     _assert_bitexact(got, seq, "batch vs sequential")
+    _case("48 rooms in groups that fit the Infinity Cache")
+    grouped, _, groups = batch.run_grouped(ws, steps)
+    assert len(groups) > 1 and sorted(i for g in groups for i in g) == list(range(48))
+    _assert_bitexact(grouped, seq, "cache groups vs sequential")
     # the impulse reaches every receiver of most rooms within the run
     reached = np.abs(got).max(axis=1) > 0
     assert reached.mean() > 0.5
