@@ -188,7 +188,7 @@ struct afx_sim {
     int rot[4] = {0, 1, 2, 3};
     DBuf d_rooms, d_tiles, d_colmask, d_pml_rec, d_pml_side;
     RoomDev room0{};
-    DBuf d_work, d_e_pack, d_pml_nbd;
+    DBuf d_work, d_e_pack, d_pml_nbd, d_pml_blk, d_pml_bmask;
     std::vector<std::pair<int64_t, int32_t>> by_cell;  // (cell, entry) sorted by cell
     int32_t nwork = 0;
     DBuf d_work1, d_work2;  // the two launches of a two-step pass
@@ -455,6 +455,8 @@ std::vector<std::array<int, 7>> pml_box_list(const afx_sim* s)
 
 // Cells holding sources or receivers.  Each is finished by one thread that
 // computes the step value, adds the sources in order and records receivers.
+bool use_fused(const afx_sim* s);
+
 int build_special(afx_sim* s)
 {
     struct Sp {
@@ -580,6 +582,64 @@ int build_special(afx_sim* s)
                     mask[(size_t)(c >> 5)] |= 1u << (c & 31);
                 }
     }
+    // Fused co-scheduled path: the PML region is stepped by z-marching PML
+    // blocks (pml_block); only its source/receiver cells keep a record, from
+    // which their pressure is finished.
+    const bool pml_blocks = s->pml_mode == 2 && use_fused(s) &&
+                            !(s->cfg.flags & (AFX_LIST_BLOCKS | AFX_PML_RECORDS));
+    std::vector<int4> blk;        // {box, packed origin, ex | ey << 8 | ez << 18, mask offset}
+    std::vector<uint32_t> bmask;  // per block column: bit dz set = no pressure there
+    std::vector<int32_t> blk_room;
+    if (pml_blocks) {
+        size_t keep = 0;
+        for (size_t k = 0; k < pcell.size(); ++k)
+            if ((int)((uint32_t)pflag[k] & 0xFFFFFFu) - 2 >= 0) {
+                pcell[keep] = pcell[k];
+                pxyz[keep] = pxyz[k];
+                pflag[keep] = pflag[k];
+                ++keep;
+            }
+        pcell.resize(keep);
+        pxyz.resize(keep);
+        pflag.resize(keep);
+        const TileShape ts = kTiles[s->tile];
+        const int NT = ts.tx / ts.vec * ts.ty;
+        constexpr int kBlockZ = 16;
+        const auto boxes = pml_box_list(s);
+        auto no_pressure = [&](int64_t c) {
+            auto it = std::lower_bound(s->by_cell.begin(), s->by_cell.end(), std::make_pair(c, (int32_t)-1));
+            if (it != s->by_cell.end() && it->first == c) return true;  // boundary cell: LRS update
+            auto q = std::lower_bound(pml_sp.begin(), pml_sp.end(), std::make_pair(c, (int32_t)-1));
+            return q != pml_sp.end() && q->first == c;                  // source/receiver cell
+        };
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            const auto& q = boxes[b];
+            const Room& r = s->rooms[(size_t)q[0]];
+            const int wx = q[2] - q[1], wy = q[4] - q[3];
+            const int tw = std::min(wx, 64), th = std::max(1, std::min(wy, NT / tw));
+            for (int z0 = q[5]; z0 < q[6]; z0 += kBlockZ)
+                for (int y0 = q[3]; y0 < q[4]; y0 += th)
+                    for (int x0 = q[1]; x0 < q[2]; x0 += tw) {
+                        const int ex = std::min(tw, q[2] - x0), ey = std::min(th, q[4] - y0);
+                        const int ez = std::min(kBlockZ, q[6] - z0);
+                        blk.push_back(make_int4((int)b, (int)pack_xyz(x0, y0, z0), ex | (ey << 8) | (ez << 18),
+                                                (int)bmask.size()));
+                        blk_room.push_back(q[0]);
+                        for (int ty = 0; ty < ey; ++ty)
+                            for (int tx = 0; tx < ex; ++tx) {
+                                uint32_t bits = 0;
+                                for (int dz = 0; dz < ez; ++dz) {
+                                    const int64_t c = r.base + ((int64_t)(z0 + dz) * r.ny + (y0 + ty)) * r.nx + x0 + tx;
+                                    if (no_pressure(c)) bits |= 1u << dz;
+                                }
+                                bmask.push_back(bits);
+                            }
+                    }
+        }
+        if (bmask.size() >= (size_t)INT32_MAX) return fail(AFX_E_RANGE, "PML blocks too large");
+    }
+    HIP_TRY(upload(s->d_pml_blk, blk, s->stream));
+    HIP_TRY(upload(s->d_pml_bmask, bmask, s->stream));
     // Work table of the fused kernel.  Boundary entries (packed records; the
     // IIR rings stay in entry order) and PML cells (the list itself) are
     // grouped by the interior tile that holds them, in XCD-major tile order.
@@ -596,8 +656,23 @@ int build_special(afx_sim* s)
         const TileRanker ranker(s);
         const int64_t per = ranker.per;
         auto rank_of_cell = [&](int room, int x, int y, int z) { return ranker.rank(room, x, y, z); };
-        std::vector<int32_t> bcount(nslot + 1, 0), pcount(nslot + 1, 0);
+        std::vector<int32_t> bcount(nslot + 1, 0), pcount(nslot + 1, 0), kcount(nslot + 1, 0);
         std::vector<int32_t> brank(s->entries.size()), prank(pcell.size());
+        // PML blocks by the tile holding their first cell
+        std::vector<int32_t> kfirst;
+        {
+            std::vector<int32_t> krank(blk.size());
+            for (size_t b = 0; b < blk.size(); ++b) {
+                const uint32_t o = (uint32_t)blk[b].y;
+                krank[b] = (int32_t)rank_of_cell(blk_room[b], (int)(o & 2047u), (int)((o >> 11) & 2047u),
+                                                 (int)(o >> 22));
+                kcount[(size_t)krank[b] + 1]++;
+            }
+            for (size_t i = 0; i < nslot; ++i) kcount[i + 1] += kcount[i];
+            kfirst.assign(blk.size(), 0);
+            std::vector<int32_t> kf(kcount.begin(), kcount.end() - 1);
+            for (size_t b = 0; b < blk.size(); ++b) kfirst[(size_t)kf[(size_t)krank[b]]++] = (int32_t)b;
+        }
         for (size_t e = 0; e < s->entries.size(); ++e) {
             const Entry& en = s->entries[e];
             brank[e] = (int32_t)rank_of_cell(en.room, en.x, en.y, en.z);
@@ -677,7 +752,9 @@ int build_special(afx_sim* s)
                     for (; bcur < bcount[(size_t)rk + 1]; bcur += NT)
                         v.push_back(make_int4(1, bcur, std::min<int32_t>(bcur + NT, bend), 0));
                     for (; pcur < pcount[(size_t)rk + 1]; pcur += NT)
-                        v.push_back(make_int4(2, pcur, std::min<int32_t>(pcur + NT, pend), 0));
+                        v.push_back(make_int4(pml_blocks ? 6 : 2, pcur, std::min<int32_t>(pcur + NT, pend), 0));
+                    for (int32_t q = kcount[(size_t)rk]; q < kcount[(size_t)rk + 1]; ++q)
+                        v.push_back(make_int4(5, kfirst[(size_t)q], 0, 0));
                     const int4 t4 = table[(size_t)sl];
                     if (t4.x < 0 || (mode == 2 && pair_ok[(size_t)sl])) continue;
                     const int kind = mode == 1 && pair_ok[(size_t)sl] ? 4 : 0;
@@ -697,7 +774,7 @@ int build_special(afx_sim* s)
         std::vector<int32_t> chunk_of(pcell.size(), -1);
         for (const auto& v : seq)
             for (const int4& w : v)
-                if (w.x == 2)
+                if (w.x == 2 || w.x == 6)
                     for (int q = w.y; q < w.z; ++q) chunk_of[(size_t)q] = w.y;
         std::unordered_map<int64_t, int32_t> pos_of;
         pos_of.reserve(pcell.size() * 2);
@@ -1230,6 +1307,8 @@ StepArgs make_args(afx_sim* s)
     a.nwork = cosched ? s->nwork : 0;
     a.e_pack = s->d_e_pack.as<int4>();
     a.pml_nbd = s->d_pml_nbd.as<uint32_t>();
+    a.pml_blk = s->d_pml_blk.as<int4>();
+    a.pml_bmask = s->d_pml_bmask.as<uint32_t>();
     a.colmask = s->d_colmask.as<uint32_t>();
     a.dt = s->cfg.dt;
     a.dx = s->cfg.dx;

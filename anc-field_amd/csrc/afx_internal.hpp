@@ -108,6 +108,11 @@ struct StepArgs {
     const uint32_t* pml_side;   // box | neighbours in other boxes << 26, for cells flagged in pml_rec
     const int32_t* pml_boxes;   // [nbox][8]: room, x0, x1, y0, y1, z0, z1, compact base
     int32_t npml;
+    // fused path, PML blocks: {box, packed origin, ex | ey << 8 | ez << 18,
+    // mask offset} per block and one skip mask per block column (bit dz:
+    // the pressure of that cell is left to another thread)
+    const int4* pml_blk;
+    const uint32_t* pml_bmask;
     // fused path work table (kind, a, b) per workgroup, XCD-interleaved:
     // interior tiles with the list chunks holding their cells (boundary
     // entries as packed records {cell, info, mats, entry} grouped by tile;

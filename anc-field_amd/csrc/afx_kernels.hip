@@ -641,6 +641,8 @@ struct BndCtx {
     const int4* pml_rec;
     const uint32_t* pml_side;
     const int32_t* pml_boxes;
+    const int4* pml_blk;
+    const uint32_t* pml_bmask;
 };
 
 __device__ __forceinline__ void unpack_xyz(uint32_t p, int& x, int& y, int& z)
@@ -790,7 +792,7 @@ __device__ __forceinline__ int64_t pml_nb(const BndCtx<S, C>& c, const PmlCell<C
                     u.y + ((dir >> 1) == 1 ? sgn : 0), u.z + ((dir >> 1) == 2 ? sgn : 0));
 }
 
-template <typename S, typename C>
+template <typename S, typename C, bool kStore = true>
 __device__ __forceinline__ void pml_first(const BndCtx<S, C>& c, int q, int par, PmlCell<C>& u)
 {
     const int4 rec = c.pml_rec[q];
@@ -810,13 +812,15 @@ __device__ __forceinline__ void pml_first(const BndCtx<S, C>& c, int q, int par,
     u.fx = phi_at<S, C>(c, u.r, 0, u.i, u.x, u.y, u.z, u.k, kxp, c.phib[par ^ 1][0], psr, 1);
     u.fy = phi_at<S, C>(c, u.r, 1, u.i, u.x, u.y, u.z, u.k, kyp, c.phib[par ^ 1][1], psr, sy);
     u.fz = phi_at<S, C>(c, u.r, 2, u.i, u.x, u.y, u.z, u.k, kzp, c.phib[par ^ 1][2], psr, sz);
-    c.phib[par][0][u.k] = u.fx;
-    c.phib[par][1][u.k] = u.fy;
-    c.phib[par][2][u.k] = u.fz;
+    if (kStore) {
+        c.phib[par][0][u.k] = u.fx;
+        c.phib[par][1][u.k] = u.fy;
+        c.phib[par][2][u.k] = u.fz;
+    }
     u.pc = ld<S, C>(c.cur, u.i);
     u.ps = psr[u.k];
     u.psn = u.ps + c.pml.dt * u.pc;
-    c.psib[par][u.k] = u.psn;
+    if (kStore) c.psib[par][u.k] = u.psn;
 }
 
 template <typename S, typename C>
@@ -864,7 +868,7 @@ __device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int q, int64_t 
 // thread shares its phi^n through LDS; a neighbour's phi^n is read from there
 // when the neighbour is in the chunk (nbd[q]: list-index distances, 10 bits
 // per axis, 0 = not in the chunk) and recomputed otherwise.
-template <typename S, typename C, int NT>
+template <typename S, typename C, int NT, bool kStore = true>
 __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, int64_t t,
                                           const uint32_t* __restrict__ nbd, C (*lds)[NT])
 {
@@ -875,7 +879,7 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
     PmlCell<C> u;
     u.flag = -2;
     if (valid) {
-        pml_first<S, C>(c, q, par, u);
+        pml_first<S, C, kStore>(c, q, par, u);
         lds[0][tid] = u.fx;
         lds[1][tid] = u.fy;
         lds[2][tid] = u.fz;
@@ -886,6 +890,100 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
     const int dxm = (int)(nb & 1023u), dym = (int)((nb >> 10) & 1023u), dzm = (int)(nb >> 20);
     pml_second<S, C>(c, u, par, t, dxm ? lds[0][tid - dxm] : (C)0, dym ? lds[1][tid - dym] : (C)0,
                      dzm ? lds[2][tid - dzm] : (C)0, dxm != 0, dym != 0, dzm != 0);
+}
+
+// One PML block of the fused launch: an ex x ey block of columns of one PML
+// box, each thread one column marching ez planes of z.  Per plane every
+// thread computes phi^n (three components) and psi^{n+1/2} at its cell and
+// stores them, exactly as pml_first does from a record; after one barrier it
+// takes phi^n of its -x and -y neighbours from the threads that computed them
+// (LDS, two planes double-buffered), of its -z neighbour from its own
+// previous plane, and recomputes only those that lie outside the block, as
+// pml_second does.  Cells whose bit dz is set in the column's skip mask
+// (boundary cells, whose pressure is the LRS update, and source/receiver
+// cells, finished from their record) get phi and psi but no pressure here.
+template <typename S, typename C, int NT>
+__device__ __forceinline__ void pml_block(const BndCtx<S, C>& c, int blk, int64_t t, C* __restrict__ lds)
+{
+    const int4 bd = c.pml_blk[blk];
+    const int32_t* B = c.pml_boxes + 8 * bd.x;
+    const RoomDev r = c.rooms[B[0]];
+    int ox, oy, oz;
+    unpack_xyz((uint32_t)bd.y, ox, oy, oz);
+    const int ex = bd.z & 255, ey = (bd.z >> 8) & 1023, ez = (int)((uint32_t)bd.z >> 18);
+    const int tid = (int)threadIdx.x;
+    const int tx = tid % ex, ty = tid / ex;
+    const bool active = ty < ey;
+    const int x = ox + tx, y = oy + (active ? ty : 0);
+    const int wx = B[2] - B[1], wy = B[4] - B[3];
+    const int64_t wxy = (int64_t)wx * wy;
+    const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+    int64_t k = (int64_t)B[7] + ((int64_t)(oz - B[5]) * wy + (y - B[3])) * wx + (x - B[1]);
+    int64_t i = r.base + (int64_t)oz * sz + (int64_t)y * sy + x;
+    const uint32_t skip = active ? c.pml_bmask[bd.w + tid] : ~0u;
+    const int par = (int)(t & 1);
+    const C* psr = c.psib[par ^ 1];
+    const C* phx = c.phib[par ^ 1][0];
+    const C* phy = c.phib[par ^ 1][1];
+    const C* phz = c.phib[par ^ 1][2];
+    C fzc = (C)0;  // phi_z^n one plane down in this column
+    for (int dz = 0; dz < ez; ++dz) {
+        const int z = oz + dz;
+        C* L = lds + (dz & 1) * 2 * NT;
+        C fx = (C)0, fy = (C)0, fz = (C)0, pc = (C)0, ps = (C)0, psn = (C)0;
+        if (active) {
+            const int64_t kxp = x + 1 < B[2] ? k + 1 : pml_find(c.pml_boxes, r, bd.x, x + 1, y, z);
+            const int64_t kyp = y + 1 < B[4] ? k + wx : pml_find(c.pml_boxes, r, bd.x, x, y + 1, z);
+            const int64_t kzp = z + 1 < B[6] ? k + wxy : pml_find(c.pml_boxes, r, bd.x, x, y, z + 1);
+            fx = phi_at<S, C>(c, r, 0, i, x, y, z, k, kxp, phx, psr, 1);
+            fy = phi_at<S, C>(c, r, 1, i, x, y, z, k, kyp, phy, psr, sy);
+            fz = phi_at<S, C>(c, r, 2, i, x, y, z, k, kzp, phz, psr, sz);
+            c.phib[par][0][k] = fx;
+            c.phib[par][1][k] = fy;
+            c.phib[par][2][k] = fz;
+            pc = ld<S, C>(c.cur, i);
+            ps = psr[k];
+            psn = ps + c.pml.dt * pc;
+            c.psib[par][k] = psn;
+            L[tid] = fx;
+            L[NT + tid] = fy;
+        }
+        __syncthreads();
+        if (active && !((skip >> dz) & 1u)) {
+            C fxm, fym, fzm;
+            if (tx > 0) {
+                fxm = L[tid - 1];
+            } else {
+                const int64_t kxm = x - 1 >= B[1] ? k - 1 : pml_find(c.pml_boxes, r, bd.x, x - 1, y, z);
+                fxm = phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, kxm, k, phx, psr, 1);
+            }
+            if (ty > 0) {
+                fym = L[NT + tid - ex];
+            } else {
+                const int64_t kym = y - 1 >= B[3] ? k - wx : pml_find(c.pml_boxes, r, bd.x, x, y - 1, z);
+                fym = phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, kym, k, phy, psr, sy);
+            }
+            if (dz > 0) {
+                fzm = fzc;
+            } else {
+                const int64_t kzm = z - 1 >= B[5] ? k - wxy : pml_find(c.pml_boxes, r, bd.x, x, y, z - 1);
+                fzm = phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, kzm, k, phz, psr, sz);
+            }
+            const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
+            C sum = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
+            sum = sum + ld<S, C>(c.cur, i + sy);
+            sum = sum + ld<S, C>(c.cur, i - sy);
+            sum = sum + ld<S, C>(c.cur, i + sz);
+            sum = sum + ld<S, C>(c.cur, i - sz);
+            const C v = pml_formula<C>(sum, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
+                                       c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt,
+                                       c.pml.dx, c.k.l2);
+            st<S, C>(c.out, i, v);
+        }
+        fzc = fz;
+        k += wxy;
+        i += sz;
+    }
 }
 
 template <typename S, typename C>
@@ -950,6 +1048,8 @@ __host__ BndCtx<S, C> make_ctx(const StepArgs& a)
     c.pml_rec = a.pml_rec;
     c.pml_side = a.pml_side;
     c.pml_boxes = a.pml_boxes;
+    c.pml_blk = a.pml_blk;
+    c.pml_bmask = a.pml_bmask;
     return c;
 }
 
@@ -993,12 +1093,14 @@ __global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Comput
     constexpr int NT = TX / V * TY;   // threads
     constexpr int NCOL = TX * TY;     // columns (column-mask words per tile slot)
     constexpr int kPair = V == 1 ? PairLds<TX, TY>::size : 0;
-    constexpr int kLds = kPair > 3 * NT ? kPair : 3 * NT;
+    constexpr int kLds = kPair > 4 * NT ? kPair : 4 * NT;
     __shared__ C lds[kLds];  // PML chunks: phi^n by thread; two-step tiles: pressure planes
     // Work table: item (kind, a, b, w).  kind 0: interior tile of slot a,
     // origin packed in b (x | y << 11 | z << 22), room w; 4: the same tile
     // advanced two steps (interior_pair); 1: packed boundary entries
-    // e_pack[a..b); 2: PML list [a..b); 3: special cells [a..b); -1: idle.
+    // e_pack[a..b); 2: PML list [a..b); 3: special cells [a..b); 5: PML
+    // block a (pml_block); 6: PML list [a..b) of source/receiver cells
+    // whose phi and psi a PML block stores; -1: idle.
     // Items of one XCD (blockIdx % 8) follow its tiles in order, each list
     // chunk right before the tile holding its first cell, so both read the
     // same planes of p^n through L2.
@@ -1025,6 +1127,14 @@ __global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Comput
     const int64_t t = *d_t + t_rel;
     if (wk.x == 2) {  // whole workgroup: it meets a barrier
         pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, reinterpret_cast<C(*)[NT]>(lds));
+        return;
+    }
+    if (wk.x == 5) {  // whole workgroup: barriers every plane
+        pml_block<S, C, NT>(c, wk.y, t, lds);
+        return;
+    }
+    if (wk.x == 6) {  // PML source/receiver cells next to PML blocks, which store their phi and psi
+        pml_chunk<S, C, NT, false>(c, wk.y, wk.z, t, pml_nbd, reinterpret_cast<C(*)[NT]>(lds));
         return;
     }
     const int q = wk.y + (int)threadIdx.x;
