@@ -26,7 +26,7 @@ COMPAT = 1
 NO_GRAPH = 2
 SPLIT = 4
 LIST_BLOCKS = 8
-PML_RECORDS = 16
+PML_BLOCKS = 16
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = 1, 2, 4, 8, 16, 32
 
 
@@ -179,7 +179,7 @@ class Sim:
 
     def __init__(self, c=None, dx=None, dt=None, res=None, precision="fp32", compat=True, device=0,
                  graph=True, fused=True, pml_order=None, pml_r0=None, scale=None, list_blocks=False,
-                 pml_records=False):
+                 pml_blocks=False):
         cfg = default_config()
         if c is not None:
             cfg.c = c
@@ -191,7 +191,7 @@ class Sim:
             cfg.res = res
         cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         cfg.flags = ((COMPAT if compat else 0) | (0 if graph else NO_GRAPH) | (0 if fused else SPLIT)
-                     | (LIST_BLOCKS if list_blocks else 0) | (PML_RECORDS if pml_records else 0))
+                     | (LIST_BLOCKS if list_blocks else 0) | (PML_BLOCKS if pml_blocks else 0))
         cfg.device = device
         if pml_order is not None:
             cfg.pml_order = pml_order

@@ -582,11 +582,11 @@ int build_special(afx_sim* s)
                     mask[(size_t)(c >> 5)] |= 1u << (c & 31);
                 }
     }
-    // Fused co-scheduled path: the PML region is stepped by z-marching PML
-    // blocks (pml_block); only its source/receiver cells keep a record, from
-    // which their pressure is finished.
-    const bool pml_blocks = s->pml_mode == 2 && use_fused(s) &&
-                            !(s->cfg.flags & (AFX_LIST_BLOCKS | AFX_PML_RECORDS));
+    // AFX_PML_BLOCKS (fused co-scheduled path): the PML region is stepped by
+    // z-marching PML blocks (pml_block); only its source/receiver cells keep
+    // a record, from which their pressure is finished.
+    const bool pml_blocks = s->pml_mode == 2 && use_fused(s) && (s->cfg.flags & AFX_PML_BLOCKS) &&
+                            !(s->cfg.flags & AFX_LIST_BLOCKS);
     std::vector<int4> blk;        // {box, packed origin, ex | ey << 8 | ez << 18, mask offset}
     std::vector<uint32_t> bmask;  // per block column: bit dz set = no pressure there
     std::vector<int32_t> blk_room;
@@ -604,7 +604,7 @@ int build_special(afx_sim* s)
         pflag.resize(keep);
         const TileShape ts = kTiles[s->tile];
         const int NT = ts.tx / ts.vec * ts.ty;
-        constexpr int kBlockZ = 16;
+        constexpr int kBlockZ = 2;
         const auto boxes = pml_box_list(s);
         auto no_pressure = [&](int64_t c) {
             auto it = std::lower_bound(s->by_cell.begin(), s->by_cell.end(), std::make_pair(c, (int32_t)-1));

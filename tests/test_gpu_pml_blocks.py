@@ -1,5 +1,5 @@
-"""The physical PML stepped by z-marching PML blocks (the fused kernel's
-default) against the per-cell PML records (AFX_PML_RECORDS) and the oracle,
+"""The physical PML stepped by z-marching PML blocks (AFX_PML_BLOCKS)
+against the per-cell PML records (the fused kernel's default) and the oracle,
 bit for bit: PML on several faces (boxes meeting at edges and corners),
 boundary cells and furniture inside the layer, sources and receivers in it,
 every storage precision, one- and several-cell-per-lane tiles, two steps
@@ -68,8 +68,8 @@ def test_pml_blocks_match_records_and_oracle(precision, tile, tblock):
     src = np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 800.0, amp=0.5),
                     rooms.ricker(steps, dt, 1300.0, amp=0.7)])
     mc = np.concatenate([inner[1:], np.array([[68, 30, 17], [70, 42, 38], [2, 2, 37], [40, 40, 36]], np.int32)])
-    got, field = _run(v, a, b, sc, src, mc, steps, precision, width, faces, prof, tile, tblock)
-    rec, rfield = _run(v, a, b, sc, src, mc, steps, precision, width, faces, prof, tile, tblock, pml_records=True)
+    got, field = _run(v, a, b, sc, src, mc, steps, precision, width, faces, prof, tile, tblock, pml_blocks=True)
+    rec, rfield = _run(v, a, b, sc, src, mc, steps, precision, width, faces, prof, tile, tblock)
     assert np.abs(got).max() > 0
     assert np.array_equal(got, rec), "traces differ from the record path"
     assert np.array_equal(field, rfield), "fields differ from the record path"
@@ -95,7 +95,7 @@ def test_pml_blocks_batched_rooms_match_single_runs():
         s.close()
         return tr
 
-    got = one(ws)
-    assert np.array_equal(got, one(ws, pml_records=True))
+    got = one(ws, pml_blocks=True)
+    assert np.array_equal(got, one(ws))
     for k, w in enumerate(ws):
-        assert np.array_equal(got[4 * k:4 * (k + 1)], one([w]))
+        assert np.array_equal(got[4 * k:4 * (k + 1)], one([w], pml_blocks=True))
