@@ -150,6 +150,13 @@ def batched_leg(steps):
             "gcell_per_s": round(cells * steps / t / 1e9, 1), "autotuned_tile": list(tile[:3]), "steps_per_pass": int(tile[3]),
             "rel_l2_vs_fp64": batch.rel_l2(tr, ref)}
     res["batched_fp32"]["bitexact_vs_sequential"] = bool((runs["fp32"][0] == seq).all())
+    # how the reduced-precision error grows with the run length: relative L2
+    # against FP64 over the first n steps of every trace
+    import numpy as np
+    for prec in ("fp32", "fp16", "bf16"):
+        tr = np.asarray(runs[prec][0], np.float64)
+        res[f"batched_{prec}"]["rel_l2_vs_fp64_first_steps"] = {
+            str(n): batch.rel_l2(tr[:, :n], ref[:, :n]) for n in (256, 512, 1024, 2048, 4096) if n <= steps}
     # the same batch stepped in groups whose fields fit the Infinity Cache
     for prec in ("fp32", "fp16"):
         tr, t, groups = batch.run_grouped(ws, steps, prec, tune=True)
