@@ -486,7 +486,14 @@ __global__ void k_mat_consts(const C* __restrict__ A, const C* __restrict__ B, i
 // New pressure of boundary entry e (rounded to storage precision) with its
 // IIR rings advanced.  Mirrored neighbour sum: axes x, y, z in order; a
 // solid face replaces its pair by 2*opposite (lrs_1.cl:135-175).  ixb is the
-// step's ring index, below N (ring_index).
+// step's ring index, below N (ring_index).  Memory order: p^n at the cell and
+// its six neighbours (boundary cells lie in [1, n-2] on every axis, so all
+// six are cells of the room) and p^{n-1} at the cell are requested first,
+// then each material's coefficients and ring slots kRingBatch at a time, so
+// a material's ring reads are in flight together; the arithmetic and its
+// order are those of the reference's update.
+constexpr int kRingBatch = 4;
+
 template <typename S, typename C>
 __device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* __restrict__ cur,
                        const S* __restrict__ prev, C* __restrict__ rin, C* __restrict__ rout, int64_t rs,
@@ -496,6 +503,15 @@ __device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* 
     const uint32_t m = info & kInfoMask;
     const int K = (int)((info >> kInfoKShift) & 3u);
     const unsigned sf = (compat && K == 3) ? (unsigned)N / 2u : (unsigned)N;  // lrs_3.cl:70
+    const int64_t st3[3] = {1, sy, sz};
+    const C pc = ld<S, C>(cur, i);
+    const C pd1 = ld<S, C>(prev, i);
+    C np[3], nm[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        np[d] = ld<S, C>(cur, i + st3[d]);
+        nm[d] = ld<S, C>(cur, i - st3[d]);
+    }
     C b0[3], g[3], mq[3][kMatConsts];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -507,43 +523,56 @@ __device__ C bnd_value(int e, int64_t i, uint32_t info, uint32_t mats, const S* 
 #pragma unroll
         for (int u = 0; u < kMatConsts; ++u) mq[q][u] = M[kMatConsts * mat + u];
         C gn = 0;
-        for (int ii = 1; ii < N; ++ii) {
-            // (ixb + ii) mod sf; ixb + ii < 2N, so one subtraction when sf == N
-            unsigned t = ixb + (unsigned)ii;
-            if (sf == (unsigned)N)
-                t = t >= sf ? t - sf : t;
-            else
-                t = t % sf;
-            const unsigned slot = t + (unsigned)q * sf;
-            gn += bk[ii] * rin[slot * rs + e] - ak[ii] * rout[slot * rs + e];
+        for (int i0 = 1; i0 < N; i0 += kRingBatch) {
+            C vb[kRingBatch], vi[kRingBatch], va[kRingBatch], vo[kRingBatch];
+#pragma unroll
+            for (int w = 0; w < kRingBatch; ++w) {
+                const int ii = i0 + w;
+                // (ixb + ii) mod sf; ixb + ii < 2N, so one subtraction when sf == N
+                unsigned t = ixb + (unsigned)(ii < N ? ii : 1);
+                if (sf == (unsigned)N)
+                    t = t >= sf ? t - sf : t;
+                else
+                    t = t % sf;
+                const int64_t at = (int64_t)(t + (unsigned)q * sf) * rs + e;
+                const int iic = ii < N ? ii : 0;
+                vb[w] = bk[iic];
+                vi[w] = rin[at];
+                va[w] = ak[iic];
+                vo[w] = rout[at];
+            }
+#pragma unroll
+            for (int w = 0; w < kRingBatch; ++w)
+                if (i0 + w < N) gn += vb[w] * vi[w] - va[w] * vo[w];
         }
         g[q] = gn;
     }
     const C l2 = k.l2b;
-    const C pc = ld<S, C>(cur, i);
-    const C pd1 = ld<S, C>(prev, i);
     C v;
     if (info & kInfoNoUpdate) {
-        v = leapfrog_value<S, C>(cur, prev, i, sy, sz, k.l2, k.c2);
+        // leapfrog_value's operations
+        C s = np[0] + nm[0];
+        s = s + np[1];
+        s = s + nm[1];
+        s = s + np[2];
+        s = s + nm[2];
+        const C lv = k.l2 * s + k.c2 * pc;
+        v = lv - pd1;
     } else {
-        const int64_t st3[3] = {1, sy, sz};
         C acc = 0;
-        bool first = true;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const uint32_t plus = 1u << (2 * d), minus = 2u << (2 * d);
             if (m & plus) {
-                const C tt = (C)2.0 * ld<S, C>(cur, i - st3[d]);
-                acc = first ? tt : acc + tt;
+                const C tt = (C)2.0 * nm[d];
+                acc = d == 0 ? tt : acc + tt;
             } else if (m & minus) {
-                const C tt = (C)2.0 * ld<S, C>(cur, i + st3[d]);
-                acc = first ? tt : acc + tt;
+                const C tt = (C)2.0 * np[d];
+                acc = d == 0 ? tt : acc + tt;
             } else {
-                const C up = ld<S, C>(cur, i + st3[d]);
-                acc = first ? up : acc + up;
-                acc = acc + ld<S, C>(cur, i - st3[d]);
+                acc = d == 0 ? np[d] : acc + np[d];
+                acc = acc + nm[d];
             }
-            first = false;
         }
         C num, den;
         // mq[q][0] = l2b / b0, mq[q][1] = lam_b * a0 / b0
@@ -657,7 +686,8 @@ __device__ __forceinline__ void boundary_entry(const BndCtx<S, C>& c, int e, uns
 {
     const uint32_t info = c.e_info[e];
     if (skip_special && (info & kInfoSpecial)) return;
-    const RoomDev r = c.rooms[info >> kInfoRoomShift];
+    const uint32_t room = info >> kInfoRoomShift;
+    const RoomDev r = room == 0 ? c.room0 : c.rooms[room];
     const int64_t i = c.e_cell[e];
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     const C v = bnd_value<S, C>(e, i, info, c.e_mat[e], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.M, c.N, ixb,
@@ -671,7 +701,8 @@ __device__ __forceinline__ void boundary_packed(const BndCtx<S, C>& c, int4 en, 
 {
     const uint32_t info = (uint32_t)en.y;
     if (info & kInfoSpecial) return;
-    const RoomDev r = c.rooms[info >> kInfoRoomShift];
+    const uint32_t room = info >> kInfoRoomShift;
+    const RoomDev r = room == 0 ? c.room0 : c.rooms[room];
     const int64_t i = en.x;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     const C v = bnd_value<S, C>(en.w, i, info, (uint32_t)en.z, c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.M, c.N, ixb,
@@ -711,7 +742,8 @@ __device__ __forceinline__ void special_entry(const BndCtx<S, C>& c, int j, unsi
     C v;
     if (full && b >= 0) {
         const uint32_t info = c.e_info[b];
-        const RoomDev r = c.rooms[info >> kInfoRoomShift];
+        const uint32_t room = info >> kInfoRoomShift;
+        const RoomDev r = room == 0 ? c.room0 : c.rooms[room];
         v = bnd_value<S, C>(b, i, info, c.e_mat[b], c.cur, c.prev, c.rin, c.rout, c.rs, c.A, c.B, c.M, c.N, ixb,
                             c.compat, (int64_t)r.nx, (int64_t)r.nx * r.ny, c.k);
     } else if (full && b == -1) {
@@ -867,7 +899,13 @@ __device__ __forceinline__ void pml_entry(const BndCtx<S, C>& c, int q, int64_t 
 // PML cells [a, b) of one work chunk, one per thread of the workgroup.  Each
 // thread shares its phi^n through LDS; a neighbour's phi^n is read from there
 // when the neighbour is in the chunk (nbd[q]: list-index distances, 10 bits
-// per axis, 0 = not in the chunk) and recomputed otherwise.
+// per axis, 0 = not in the chunk) and recomputed otherwise.  The arithmetic
+// is pml_first's and pml_second's; the order of the memory accesses is not:
+// every value the cell needs (its own phi^n inputs, those of the -x, -y, -z
+// neighbours, the 7-point sum of p^n) is requested in one round, with no
+// load behind a branch or a store, before anything is computed or stored.
+// Indexes that point outside the PML region (compact index -1) are replaced
+// by the cell's own for the load and the value is then discarded.
 template <typename S, typename C, int NT, bool kStore = true>
 __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, int64_t t,
                                           const uint32_t* __restrict__ nbd, C (*lds)[NT])
@@ -876,111 +914,240 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
     const int q = a + tid;
     const bool valid = q < b;
     const int par = (int)(t & 1);
-    PmlCell<C> u;
-    u.flag = -2;
+    const C* psr = c.psib[par ^ 1];
+    int flag = -2;
+    int64_t i = 0;
+    C fx = 0, fy = 0, fz = 0, rxm = 0, rym = 0, rzm = 0;
+    C sum = 0, pc = 0, pm = 0, ps = 0, psn = 0, sx = 0, syy = 0, szz = 0;
+    uint32_t nbq = 0;
     if (valid) {
-        pml_first<S, C, kStore>(c, q, par, u);
-        lds[0][tid] = u.fx;
-        lds[1][tid] = u.fy;
-        lds[2][tid] = u.fz;
+        PmlCell<C> u;
+        const int4 rec = c.pml_rec[q];
+        nbq = nbd[q];
+        const uint32_t word = (uint32_t)rec.y;
+        const uint32_t room = word >> 24;
+        u.q = q;
+        u.r = room == 0 ? c.room0 : c.rooms[room];
+        flag = (int)(word & 0xFFFFFFu) - 2;
+        unpack_xyz((uint32_t)rec.x, u.x, u.y, u.z);
+        u.k = (uint32_t)rec.z;
+        u.nb = (uint32_t)rec.w;
+        const RoomDev& r = u.r;
+        const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
+        const int64_t st[3] = {1, sy, sz};
+        i = r.base + (int64_t)u.z * sz + (int64_t)u.y * sy + u.x;
+        const int64_t k = u.k;
+        int64_t kp[3], km[3];  // compact indexes of the + and - neighbours
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            kp[d] = pml_nb<S, C>(c, u, 2 * d);
+            km[d] = pml_nb<S, C>(c, u, 2 * d + 1);
+        }
+        // ---- loads
+        const C* zc[3] = {c.pml.z[0] + r.axis_off[0], c.pml.z[1] + r.axis_off[1], c.pml.z[2] + r.axis_off[2]};
+        const int co[3] = {u.x, u.y, u.z};
+        C zeta[3], hp[3], hm[3], oldp[3], oldm[3], np1[3], nm1[3], npv[3], nmv[3], psp[3], psm[3];
+        pc = ld<S, C>(c.cur, i);
+        pm = ld<S, C>(c.prev, i);
+        ps = psr[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const C* ph = c.phib[par ^ 1][d];
+            const int64_t kpd = kp[d] >= 0 ? kp[d] : k;
+            const int64_t kmd = km[d] >= 0 ? km[d] : k;
+            zeta[d] = zc[d][co[d]];
+            hp[d] = c.zh[d][r.axis_off[d] + co[d]];
+            hm[d] = c.zh[d][r.axis_off[d] + co[d] - 1];
+            oldp[d] = ph[k];
+            oldm[d] = ph[kmd];
+            np1[d] = ld<S, C>(c.cur, i + st[d]);
+            nm1[d] = ld<S, C>(c.cur, i - st[d]);
+            npv[d] = ld<S, C>(c.prev, i + st[d]);
+            nmv[d] = ld<S, C>(c.prev, i - st[d]);
+            psp[d] = psr[kpd];
+            psm[d] = psr[kmd];
+        }
+        // ---- phi^n at the cell (phi_at with kj = k, kn = kp[d]) and at its
+        // - neighbours (kj = km[d], kn = k); g2, g3 as phi_at forms them
+        C f[3], g[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+            const C g2p = zeta[d1] + zeta[d2] - hp[d];
+            const C g2m = zeta[d1] + zeta[d2] - hm[d];
+            const C g3 = zeta[d1] * zeta[d2];
+            f[d] = phi_update<C>(oldp[d], pc, np1[d], pm, npv[d], ps, kp[d] >= 0 ? psp[d] : (C)0, hp[d], g2p, g3,
+                                 c.cc, c.idt, c.idx);
+            g[d] = km[d] < 0 ? (C)0
+                             : phi_update<C>(oldm[d], nm1[d], pc, nmv[d], pm, psm[d], ps, hm[d], g2m, g3, c.cc,
+                                             c.idt, c.idx);
+        }
+        fx = f[0];
+        fy = f[1];
+        fz = f[2];
+        rxm = g[0];
+        rym = g[1];
+        rzm = g[2];
+        psn = ps + c.pml.dt * pc;
+        sum = np1[0] + nm1[0];
+        sum = sum + np1[1];
+        sum = sum + nm1[1];
+        sum = sum + np1[2];
+        sum = sum + nm1[2];
+        sx = zeta[0];
+        syy = zeta[1];
+        szz = zeta[2];
+        // ---- stores
+        if (kStore) {
+            c.phib[par][0][k] = fx;
+            c.phib[par][1][k] = fy;
+            c.phib[par][2][k] = fz;
+            c.psib[par][k] = psn;
+        }
+        lds[0][tid] = fx;
+        lds[1][tid] = fy;
+        lds[2][tid] = fz;
     }
     __syncthreads();
-    if (!valid || u.flag == -2) return;
-    const uint32_t nb = nbd[q];
-    const int dxm = (int)(nb & 1023u), dym = (int)((nb >> 10) & 1023u), dzm = (int)(nb >> 20);
-    pml_second<S, C>(c, u, par, t, dxm ? lds[0][tid - dxm] : (C)0, dym ? lds[1][tid - dym] : (C)0,
-                     dzm ? lds[2][tid - dzm] : (C)0, dxm != 0, dym != 0, dzm != 0);
+    if (!valid || flag == -2) return;
+    const int dxm = (int)(nbq & 1023u), dym = (int)((nbq >> 10) & 1023u), dzm = (int)(nbq >> 20);
+    const C fxm = dxm ? lds[0][tid - dxm] : rxm;
+    const C fym = dym ? lds[1][tid - dym] : rym;
+    const C fzm = dzm ? lds[2][tid - dzm] : rzm;
+    const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
+    const C v = pml_formula<C>(sum, pc, pm, dv, ps, psn, sx, syy, szz, c.pml.dt, c.pml.dx, c.k.l2);
+    if (flag >= 0)
+        finish_special<S, C>(c, flag, i, t, rt<S, C>(v));
+    else
+        st<S, C>(c.out, i, v);
 }
 
-// One PML block of the fused launch: an ex x ey block of columns of one PML
-// box, each thread one column marching ez planes of z.  Per plane every
-// thread computes phi^n (three components) and psi^{n+1/2} at its cell and
-// stores them, exactly as pml_first does from a record; after one barrier it
-// takes phi^n of its -x and -y neighbours from the threads that computed them
-// (LDS, two planes double-buffered), of its -z neighbour from its own
-// previous plane, and recomputes only those that lie outside the block, as
-// pml_second does.  Cells whose bit dz is set in the column's skip mask
-// (boundary cells, whose pressure is the LRS update, and source/receiver
-// cells, finished from their record) get phi and psi but no pressure here.
+// One PML strip of the fused launch: NT consecutive columns (x, y) of one
+// PML box, one per thread in the box's x-fastest order (so phi and psi, which
+// are stored box by box in that order, are read and written as contiguous
+// runs), marching ez planes of z from z0.  Threads are independent: no LDS,
+// no barrier.  Per plane a thread computes phi^n (three components) and
+// psi^{n+1/2} at its cell and stores them, as pml_first does from a record;
+// phi^n at its -x and -y neighbours is recomputed from the same inputs the
+// neighbour's own thread uses, and at its -z neighbour is the previous
+// plane's phi_z^n (recomputed on the first plane), so the pressure is
+// pml_second's.  p^n and p^{n-1} at z-1 and z are carried in registers, and
+// the next plane's p^n, p^{n-1}, psi and phi are requested one plane ahead.
+// Cells whose bit dz is set in the column's skip mask (boundary cells, whose
+// pressure is the LRS update, and source/receiver cells, finished from their
+// record) get phi and psi but no pressure here.  Items never cross a box, so
+// the +z plane of every plane but the box's last is in the same box.
 template <typename S, typename C, int NT>
-__device__ __forceinline__ void pml_block(const BndCtx<S, C>& c, int blk, int64_t t, C* __restrict__ lds)
+__device__ __forceinline__ void pml_strip(const BndCtx<S, C>& c, int item, int64_t t)
 {
-    const int4 bd = c.pml_blk[blk];
-    const int32_t* B = c.pml_boxes + 8 * bd.x;
-    const RoomDev r = c.rooms[B[0]];
-    int ox, oy, oz;
-    unpack_xyz((uint32_t)bd.y, ox, oy, oz);
-    const int ex = bd.z & 255, ey = (bd.z >> 8) & 1023, ez = (int)((uint32_t)bd.z >> 18);
-    const int tid = (int)threadIdx.x;
-    const int tx = tid % ex, ty = tid / ex;
-    const bool active = ty < ey;
-    const int x = ox + tx, y = oy + (active ? ty : 0);
-    const int wx = B[2] - B[1], wy = B[4] - B[3];
+    const int4 d = c.pml_blk[item];  // {box, first column, z0, ez}
+    const int32_t* B = c.pml_boxes + 8 * d.x;
+    const int bx0 = B[1], bx1 = B[2], by0 = B[3], by1 = B[4], bz0 = B[5], bz1 = B[6];
+    const uint32_t room = (uint32_t)B[0];
+    const RoomDev r = room == 0 ? c.room0 : c.rooms[room];
+    const int wx = bx1 - bx0, wy = by1 - by0;
     const int64_t wxy = (int64_t)wx * wy;
+    const int tid = (int)threadIdx.x;
+    const int col0 = d.y + tid;
+    const bool active = col0 < wx * wy;
+    const int col = active ? col0 : wx * wy - 1;
+    const int cx = col % wx, cy = col / wx;
+    const int x = bx0 + cx, y = by0 + cy;
+    const int z0 = d.z, ez = d.w;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
-    int64_t k = (int64_t)B[7] + ((int64_t)(oz - B[5]) * wy + (y - B[3])) * wx + (x - B[1]);
-    int64_t i = r.base + (int64_t)oz * sz + (int64_t)y * sy + x;
-    const uint32_t skip = active ? c.pml_bmask[bd.w + tid] : ~0u;
+    int64_t k = (int64_t)B[7] + (int64_t)(z0 - bz0) * wxy + col;
+    int64_t i = r.base + (int64_t)z0 * sz + (int64_t)y * sy + x;
+    const uint32_t skip = active ? c.pml_bmask[(int64_t)item * NT + tid] : ~0u;
     const int par = (int)(t & 1);
     const C* psr = c.psib[par ^ 1];
     const C* phx = c.phib[par ^ 1][0];
     const C* phy = c.phib[par ^ 1][1];
     const C* phz = c.phib[par ^ 1][2];
-    C fzc = (C)0;  // phi_z^n one plane down in this column
+    // zeta at the column's x and y (cell centres and the +-1/2 points)
+    const C zx = c.pml.z[0][r.axis_off[0] + x], zy = c.pml.z[1][r.axis_off[1] + y];
+    const C hx = c.zh[0][r.axis_off[0] + x], hxm = c.zh[0][r.axis_off[0] + x - 1];
+    const C hy = c.zh[1][r.axis_off[1] + y], hym = c.zh[1][r.axis_off[1] + y - 1];
+    const C* zcz = c.pml.z[2] + r.axis_off[2];
+    const C* zhz = c.zh[2] + r.axis_off[2];
+    // the same box along x / y for every plane: neighbour compact offsets
+    const bool inxp = cx + 1 < wx, inxm = cx > 0, inyp = cy + 1 < wy, inym = cy > 0;
+    // p^n at z-1, z and p^{n-1} at z-1, z; phi^{n-1} of the cell at z
+    C pd = ld<S, C>(c.cur, i - sz), pc = ld<S, C>(c.cur, i);
+    C pmd = ld<S, C>(c.prev, i - sz), pm = ld<S, C>(c.prev, i);
+    C fxo = phx[k], fyo = phy[k], fzo = phz[k], ps = psr[k];
+    C fzc = (C)0;  // phi_z^n one plane down
     for (int dz = 0; dz < ez; ++dz) {
-        const int z = oz + dz;
-        C* L = lds + (dz & 1) * 2 * NT;
-        C fx = (C)0, fy = (C)0, fz = (C)0, pc = (C)0, ps = (C)0, psn = (C)0;
+        const int z = z0 + dz;
+        const bool last = dz + 1 == ez;
+        // compact indexes of the six neighbours (-1: outside the PML region)
+        const int64_t kxp = inxp ? k + 1 : pml_find(c.pml_boxes, r, d.x, x + 1, y, z);
+        const int64_t kxm = inxm ? k - 1 : pml_find(c.pml_boxes, r, d.x, x - 1, y, z);
+        const int64_t kyp = inyp ? k + wx : pml_find(c.pml_boxes, r, d.x, x, y + 1, z);
+        const int64_t kym = inym ? k - wx : pml_find(c.pml_boxes, r, d.x, x, y - 1, z);
+        const int64_t kzp = z + 1 < bz1 ? k + wxy : pml_find(c.pml_boxes, r, d.x, x, y, z + 1);
+        // ---- loads: this plane's neighbours, the next plane's own values
+        const C u = ld<S, C>(c.cur, i + sz);
+        const C xp = ld<S, C>(c.cur, i + 1), xm = ld<S, C>(c.cur, i - 1);
+        const C yp = ld<S, C>(c.cur, i + sy), ym = ld<S, C>(c.cur, i - sy);
+        const C pmu = ld<S, C>(c.prev, i + sz);
+        const C pmxp = ld<S, C>(c.prev, i + 1), pmxm = ld<S, C>(c.prev, i - 1);
+        const C pmyp = ld<S, C>(c.prev, i + sy), pmym = ld<S, C>(c.prev, i - sy);
+        const C fxom = phx[kxm >= 0 ? kxm : k], fyom = phy[kym >= 0 ? kym : k];
+        const C psxp = psr[kxp >= 0 ? kxp : k], psxm = psr[kxm >= 0 ? kxm : k];
+        const C psyp = psr[kyp >= 0 ? kyp : k], psym = psr[kym >= 0 ? kym : k];
+        const C pszp = psr[kzp >= 0 ? kzp : k];
+        const int64_t kn = last ? k : k + wxy;  // the next plane of this item
+        const C fxo_n = phx[kn], fyo_n = phy[kn], fzo_n = phz[kn];
+        const C zz = zcz[z], hz = zhz[z];
+        // ---- phi^n at the cell and at its -x, -y neighbours (phi_at's g2, g3)
+        const C fx = phi_update<C>(fxo, pc, xp, pm, pmxp, ps, kxp >= 0 ? psxp : (C)0, hx, zy + zz - hx, zy * zz,
+                                   c.cc, c.idt, c.idx);
+        const C fy = phi_update<C>(fyo, pc, yp, pm, pmyp, ps, kyp >= 0 ? psyp : (C)0, hy, zx + zz - hy, zx * zz,
+                                   c.cc, c.idt, c.idx);
+        const C fz = phi_update<C>(fzo, pc, u, pm, pmu, ps, kzp >= 0 ? pszp : (C)0, hz, zx + zy - hz, zx * zy,
+                                   c.cc, c.idt, c.idx);
+        const C fxm = kxm < 0 ? (C)0
+                              : phi_update<C>(fxom, xm, pc, pmxm, pm, psxm, ps, hxm, zy + zz - hxm, zy * zz, c.cc,
+                                              c.idt, c.idx);
+        const C fym = kym < 0 ? (C)0
+                              : phi_update<C>(fyom, ym, pc, pmym, pm, psym, ps, hym, zx + zz - hym, zx * zz, c.cc,
+                                              c.idt, c.idx);
+        C fzm = fzc;
+        if (dz == 0) {  // uniform: the plane below belongs to another item or box
+            const int64_t kzm = z - 1 >= bz0 ? k - wxy : pml_find(c.pml_boxes, r, d.x, x, y, z - 1);
+            const C fzom = phz[kzm >= 0 ? kzm : k], pszm = psr[kzm >= 0 ? kzm : k];
+            const C hzm = zhz[z - 1];
+            fzm = kzm < 0 ? (C)0
+                          : phi_update<C>(fzom, pd, pc, pmd, pm, pszm, ps, hzm, zx + zy - hzm, zx * zy, c.cc, c.idt,
+                                          c.idx);
+        }
+        const C psn = ps + c.pml.dt * pc;
+        // ---- stores
         if (active) {
-            const int64_t kxp = x + 1 < B[2] ? k + 1 : pml_find(c.pml_boxes, r, bd.x, x + 1, y, z);
-            const int64_t kyp = y + 1 < B[4] ? k + wx : pml_find(c.pml_boxes, r, bd.x, x, y + 1, z);
-            const int64_t kzp = z + 1 < B[6] ? k + wxy : pml_find(c.pml_boxes, r, bd.x, x, y, z + 1);
-            fx = phi_at<S, C>(c, r, 0, i, x, y, z, k, kxp, phx, psr, 1);
-            fy = phi_at<S, C>(c, r, 1, i, x, y, z, k, kyp, phy, psr, sy);
-            fz = phi_at<S, C>(c, r, 2, i, x, y, z, k, kzp, phz, psr, sz);
             c.phib[par][0][k] = fx;
             c.phib[par][1][k] = fy;
             c.phib[par][2][k] = fz;
-            pc = ld<S, C>(c.cur, i);
-            ps = psr[k];
-            psn = ps + c.pml.dt * pc;
             c.psib[par][k] = psn;
-            L[tid] = fx;
-            L[NT + tid] = fy;
+            if (!((skip >> dz) & 1u)) {
+                const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
+                C sum = xp + xm;
+                sum = sum + yp;
+                sum = sum + ym;
+                sum = sum + u;
+                sum = sum + pd;
+                st<S, C>(c.out, i, pml_formula<C>(sum, pc, pm, dv, ps, psn, zx, zy, zz, c.pml.dt, c.pml.dx, c.k.l2));
+            }
         }
-        __syncthreads();
-        if (active && !((skip >> dz) & 1u)) {
-            C fxm, fym, fzm;
-            if (tx > 0) {
-                fxm = L[tid - 1];
-            } else {
-                const int64_t kxm = x - 1 >= B[1] ? k - 1 : pml_find(c.pml_boxes, r, bd.x, x - 1, y, z);
-                fxm = phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, kxm, k, phx, psr, 1);
-            }
-            if (ty > 0) {
-                fym = L[NT + tid - ex];
-            } else {
-                const int64_t kym = y - 1 >= B[3] ? k - wx : pml_find(c.pml_boxes, r, bd.x, x, y - 1, z);
-                fym = phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, kym, k, phy, psr, sy);
-            }
-            if (dz > 0) {
-                fzm = fzc;
-            } else {
-                const int64_t kzm = z - 1 >= B[5] ? k - wxy : pml_find(c.pml_boxes, r, bd.x, x, y, z - 1);
-                fzm = phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, kzm, k, phz, psr, sz);
-            }
-            const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
-            C sum = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
-            sum = sum + ld<S, C>(c.cur, i + sy);
-            sum = sum + ld<S, C>(c.cur, i - sy);
-            sum = sum + ld<S, C>(c.cur, i + sz);
-            sum = sum + ld<S, C>(c.cur, i - sz);
-            const C v = pml_formula<C>(sum, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
-                                       c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt,
-                                       c.pml.dx, c.k.l2);
-            st<S, C>(c.out, i, v);
-        }
+        // ---- next plane
         fzc = fz;
+        pd = pc;
+        pc = u;
+        pmd = pm;
+        pm = pmu;
+        fxo = fxo_n;
+        fyo = fyo_n;
+        fzo = fzo_n;
+        ps = pszp;
         k += wxy;
         i += sz;
     }
@@ -1099,7 +1266,7 @@ __global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Comput
     // origin packed in b (x | y << 11 | z << 22), room w; 4: the same tile
     // advanced two steps (interior_pair); 1: packed boundary entries
     // e_pack[a..b); 2: PML list [a..b); 3: special cells [a..b); 5: PML
-    // block a (pml_block); 6: PML list [a..b) of source/receiver cells
+    // strip a (pml_strip); 6: PML list [a..b) of source/receiver cells
     // whose phi and psi a PML block stores; -1: idle.
     // Items of one XCD (blockIdx % 8) follow its tiles in order, each list
     // chunk right before the tile holding its first cell, so both read the
@@ -1129,8 +1296,8 @@ __global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Comput
         pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, reinterpret_cast<C(*)[NT]>(lds));
         return;
     }
-    if (wk.x == 5) {  // whole workgroup: barriers every plane
-        pml_block<S, C, NT>(c, wk.y, t, lds);
+    if (wk.x == 5) {  // PML strip: one column per thread, no barrier
+        pml_strip<S, C, NT>(c, wk.y, t);
         return;
     }
     if (wk.x == 6) {  // PML source/receiver cells next to PML blocks, which store their phi and psi
