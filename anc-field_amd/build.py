@@ -69,6 +69,9 @@ def build(verbose=False):
     if os.path.exists(repl_src):
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", repl_path(), repl_src,
               "-L" + LIB_DIR, "-lafx", "-Wl,-rpath,$ORIGIN/../lib"])
+    ab_src = os.path.join(CSRC, "afx_interior_ab.hip")
+    if os.path.exists(ab_src):
+        _run([HIPCC] + COMMON + DEVICE + ["-o", os.path.join(BIN_DIR, "afx-interior-ab"), ab_src])
     if verbose:
         print("built", lib_path())
     return lib_path()
