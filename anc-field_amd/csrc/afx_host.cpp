@@ -420,13 +420,6 @@ void subtract_box(const std::array<int, 6>& a, const std::array<int, 6>& b, std:
 // Disjoint boxes {room, x0, x1, y0, y1, z0, z1} covering each room's
 // physical PML region (the union of its face slabs, cells 1..n-2 on every
 // axis), room by room: each slab minus the boxes already placed.
-// Planes per PML strip (pml_strip); at most 32, the bits of a skip word.
-#ifndef AFX_STRIP_Z
-#define AFX_STRIP_Z 2
-#endif
-constexpr int kStripZ = AFX_STRIP_Z;
-static_assert(kStripZ >= 1 && kStripZ <= 32, "PML strip height");
-
 std::vector<std::array<int, 7>> pml_box_list(const afx_sim* s)
 {
     std::vector<std::array<int, 7>> boxes;
@@ -590,15 +583,13 @@ int build_special(afx_sim* s)
                 }
     }
     // AFX_PML_BLOCKS (fused co-scheduled path): the PML region is stepped by
-    // z-marching PML strips (pml_strip): NT consecutive columns of one box,
-    // kStripZ planes; only its source/receiver cells keep a record, from
-    // which their pressure is finished.
+    // z-marching PML blocks (pml_block); only its source/receiver cells keep
+    // a record, from which their pressure is finished.
     const bool pml_blocks = s->pml_mode == 2 && use_fused(s) && (s->cfg.flags & AFX_PML_BLOCKS) &&
                             !(s->cfg.flags & AFX_LIST_BLOCKS);
-    std::vector<int4> blk;        // {box, first column, z0, ez}
-    std::vector<uint32_t> bmask;  // NT words per strip: bit dz set = no pressure there
+    std::vector<int4> blk;        // {box, packed origin, ex | ey << 8 | ez << 18, mask offset}
+    std::vector<uint32_t> bmask;  // per block column: bit dz set = no pressure there
     std::vector<int32_t> blk_room;
-    std::vector<uint32_t> blk_first;  // packed first cell of each strip
     if (pml_blocks) {
         size_t keep = 0;
         for (size_t k = 0; k < pcell.size(); ++k)
@@ -613,6 +604,7 @@ int build_special(afx_sim* s)
         pflag.resize(keep);
         const TileShape ts = kTiles[s->tile];
         const int NT = ts.tx / ts.vec * ts.ty;
+        constexpr int kBlockZ = 2;
         const auto boxes = pml_box_list(s);
         auto no_pressure = [&](int64_t c) {
             auto it = std::lower_bound(s->by_cell.begin(), s->by_cell.end(), std::make_pair(c, (int32_t)-1));
@@ -624,27 +616,27 @@ int build_special(afx_sim* s)
             const auto& q = boxes[b];
             const Room& r = s->rooms[(size_t)q[0]];
             const int wx = q[2] - q[1], wy = q[4] - q[3];
-            const int ncol = wx * wy;
-            for (int z0 = q[5]; z0 < q[6]; z0 += kStripZ)
-                for (int c0 = 0; c0 < ncol; c0 += NT) {
-                    const int ez = std::min(kStripZ, q[6] - z0);
-                    blk.push_back(make_int4((int)b, c0, z0, ez));
-                    blk_room.push_back(q[0]);
-                    blk_first.push_back(pack_xyz(q[1] + c0 % wx, q[3] + c0 / wx, z0));
-                    for (int tcol = 0; tcol < NT; ++tcol) {
-                        uint32_t bits = 0;
-                        if (c0 + tcol < ncol) {
-                            const int x = q[1] + (c0 + tcol) % wx, y = q[3] + (c0 + tcol) / wx;
-                            for (int dz = 0; dz < ez; ++dz) {
-                                const int64_t c = r.base + ((int64_t)(z0 + dz) * r.ny + y) * r.nx + x;
-                                if (no_pressure(c)) bits |= 1u << dz;
+            const int tw = std::min(wx, 64), th = std::max(1, std::min(wy, NT / tw));
+            for (int z0 = q[5]; z0 < q[6]; z0 += kBlockZ)
+                for (int y0 = q[3]; y0 < q[4]; y0 += th)
+                    for (int x0 = q[1]; x0 < q[2]; x0 += tw) {
+                        const int ex = std::min(tw, q[2] - x0), ey = std::min(th, q[4] - y0);
+                        const int ez = std::min(kBlockZ, q[6] - z0);
+                        blk.push_back(make_int4((int)b, (int)pack_xyz(x0, y0, z0), ex | (ey << 8) | (ez << 18),
+                                                (int)bmask.size()));
+                        blk_room.push_back(q[0]);
+                        for (int ty = 0; ty < ey; ++ty)
+                            for (int tx = 0; tx < ex; ++tx) {
+                                uint32_t bits = 0;
+                                for (int dz = 0; dz < ez; ++dz) {
+                                    const int64_t c = r.base + ((int64_t)(z0 + dz) * r.ny + (y0 + ty)) * r.nx + x0 + tx;
+                                    if (no_pressure(c)) bits |= 1u << dz;
+                                }
+                                bmask.push_back(bits);
                             }
-                        }
-                        bmask.push_back(bits);
                     }
-                }
         }
-        if (bmask.size() >= (size_t)INT32_MAX) return fail(AFX_E_RANGE, "PML strips too large");
+        if (bmask.size() >= (size_t)INT32_MAX) return fail(AFX_E_RANGE, "PML blocks too large");
     }
     HIP_TRY(upload(s->d_pml_blk, blk, s->stream));
     HIP_TRY(upload(s->d_pml_bmask, bmask, s->stream));
@@ -671,7 +663,7 @@ int build_special(afx_sim* s)
         {
             std::vector<int32_t> krank(blk.size());
             for (size_t b = 0; b < blk.size(); ++b) {
-                const uint32_t o = blk_first[b];
+                const uint32_t o = (uint32_t)blk[b].y;
                 krank[b] = (int32_t)rank_of_cell(blk_room[b], (int)(o & 2047u), (int)((o >> 11) & 2047u),
                                                  (int)(o >> 22));
                 kcount[(size_t)krank[b] + 1]++;

@@ -1022,132 +1022,95 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
         st<S, C>(c.out, i, v);
 }
 
-// One PML strip of the fused launch: NT consecutive columns (x, y) of one
-// PML box, one per thread in the box's x-fastest order (so phi and psi, which
-// are stored box by box in that order, are read and written as contiguous
-// runs), marching ez planes of z from z0.  Threads are independent: no LDS,
-// no barrier.  Per plane a thread computes phi^n (three components) and
-// psi^{n+1/2} at its cell and stores them, as pml_first does from a record;
-// phi^n at its -x and -y neighbours is recomputed from the same inputs the
-// neighbour's own thread uses, and at its -z neighbour is the previous
-// plane's phi_z^n (recomputed on the first plane), so the pressure is
-// pml_second's.  p^n and p^{n-1} at z-1 and z are carried in registers, and
-// the next plane's p^n, p^{n-1}, psi and phi are requested one plane ahead.
-// Cells whose bit dz is set in the column's skip mask (boundary cells, whose
-// pressure is the LRS update, and source/receiver cells, finished from their
-// record) get phi and psi but no pressure here.  Items never cross a box, so
-// the +z plane of every plane but the box's last is in the same box.
+// One PML block of the fused launch: an ex x ey block of columns of one PML
+// box, each thread one column marching ez planes of z.  Per plane every
+// thread computes phi^n (three components) and psi^{n+1/2} at its cell and
+// stores them, exactly as pml_first does from a record; after one barrier it
+// takes phi^n of its -x and -y neighbours from the threads that computed them
+// (LDS, two planes double-buffered), of its -z neighbour from its own
+// previous plane, and recomputes only those that lie outside the block, as
+// pml_second does.  Cells whose bit dz is set in the column's skip mask
+// (boundary cells, whose pressure is the LRS update, and source/receiver
+// cells, finished from their record) get phi and psi but no pressure here.
 template <typename S, typename C, int NT>
-__device__ __forceinline__ void pml_strip(const BndCtx<S, C>& c, int item, int64_t t)
+__device__ __forceinline__ void pml_block(const BndCtx<S, C>& c, int blk, int64_t t, C* __restrict__ lds)
 {
-    const int4 d = c.pml_blk[item];  // {box, first column, z0, ez}
-    const int32_t* B = c.pml_boxes + 8 * d.x;
-    const int bx0 = B[1], bx1 = B[2], by0 = B[3], by1 = B[4], bz0 = B[5], bz1 = B[6];
-    const uint32_t room = (uint32_t)B[0];
-    const RoomDev r = room == 0 ? c.room0 : c.rooms[room];
-    const int wx = bx1 - bx0, wy = by1 - by0;
-    const int64_t wxy = (int64_t)wx * wy;
+    const int4 bd = c.pml_blk[blk];
+    const int32_t* B = c.pml_boxes + 8 * bd.x;
+    const RoomDev r = c.rooms[B[0]];
+    int ox, oy, oz;
+    unpack_xyz((uint32_t)bd.y, ox, oy, oz);
+    const int ex = bd.z & 255, ey = (bd.z >> 8) & 1023, ez = (int)((uint32_t)bd.z >> 18);
     const int tid = (int)threadIdx.x;
-    const int col0 = d.y + tid;
-    const bool active = col0 < wx * wy;
-    const int col = active ? col0 : wx * wy - 1;
-    const int cx = col % wx, cy = col / wx;
-    const int x = bx0 + cx, y = by0 + cy;
-    const int z0 = d.z, ez = d.w;
+    const int tx = tid % ex, ty = tid / ex;
+    const bool active = ty < ey;
+    const int x = ox + tx, y = oy + (active ? ty : 0);
+    const int wx = B[2] - B[1], wy = B[4] - B[3];
+    const int64_t wxy = (int64_t)wx * wy;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
-    int64_t k = (int64_t)B[7] + (int64_t)(z0 - bz0) * wxy + col;
-    int64_t i = r.base + (int64_t)z0 * sz + (int64_t)y * sy + x;
-    const uint32_t skip = active ? c.pml_bmask[(int64_t)item * NT + tid] : ~0u;
+    int64_t k = (int64_t)B[7] + ((int64_t)(oz - B[5]) * wy + (y - B[3])) * wx + (x - B[1]);
+    int64_t i = r.base + (int64_t)oz * sz + (int64_t)y * sy + x;
+    const uint32_t skip = active ? c.pml_bmask[bd.w + tid] : ~0u;
     const int par = (int)(t & 1);
     const C* psr = c.psib[par ^ 1];
     const C* phx = c.phib[par ^ 1][0];
     const C* phy = c.phib[par ^ 1][1];
     const C* phz = c.phib[par ^ 1][2];
-    // zeta at the column's x and y (cell centres and the +-1/2 points)
-    const C zx = c.pml.z[0][r.axis_off[0] + x], zy = c.pml.z[1][r.axis_off[1] + y];
-    const C hx = c.zh[0][r.axis_off[0] + x], hxm = c.zh[0][r.axis_off[0] + x - 1];
-    const C hy = c.zh[1][r.axis_off[1] + y], hym = c.zh[1][r.axis_off[1] + y - 1];
-    const C* zcz = c.pml.z[2] + r.axis_off[2];
-    const C* zhz = c.zh[2] + r.axis_off[2];
-    // the same box along x / y for every plane: neighbour compact offsets
-    const bool inxp = cx + 1 < wx, inxm = cx > 0, inyp = cy + 1 < wy, inym = cy > 0;
-    // p^n at z-1, z and p^{n-1} at z-1, z; phi^{n-1} of the cell at z
-    C pd = ld<S, C>(c.cur, i - sz), pc = ld<S, C>(c.cur, i);
-    C pmd = ld<S, C>(c.prev, i - sz), pm = ld<S, C>(c.prev, i);
-    C fxo = phx[k], fyo = phy[k], fzo = phz[k], ps = psr[k];
-    C fzc = (C)0;  // phi_z^n one plane down
+    C fzc = (C)0;  // phi_z^n one plane down in this column
     for (int dz = 0; dz < ez; ++dz) {
-        const int z = z0 + dz;
-        const bool last = dz + 1 == ez;
-        // compact indexes of the six neighbours (-1: outside the PML region)
-        const int64_t kxp = inxp ? k + 1 : pml_find(c.pml_boxes, r, d.x, x + 1, y, z);
-        const int64_t kxm = inxm ? k - 1 : pml_find(c.pml_boxes, r, d.x, x - 1, y, z);
-        const int64_t kyp = inyp ? k + wx : pml_find(c.pml_boxes, r, d.x, x, y + 1, z);
-        const int64_t kym = inym ? k - wx : pml_find(c.pml_boxes, r, d.x, x, y - 1, z);
-        const int64_t kzp = z + 1 < bz1 ? k + wxy : pml_find(c.pml_boxes, r, d.x, x, y, z + 1);
-        // ---- loads: this plane's neighbours, the next plane's own values
-        const C u = ld<S, C>(c.cur, i + sz);
-        const C xp = ld<S, C>(c.cur, i + 1), xm = ld<S, C>(c.cur, i - 1);
-        const C yp = ld<S, C>(c.cur, i + sy), ym = ld<S, C>(c.cur, i - sy);
-        const C pmu = ld<S, C>(c.prev, i + sz);
-        const C pmxp = ld<S, C>(c.prev, i + 1), pmxm = ld<S, C>(c.prev, i - 1);
-        const C pmyp = ld<S, C>(c.prev, i + sy), pmym = ld<S, C>(c.prev, i - sy);
-        const C fxom = phx[kxm >= 0 ? kxm : k], fyom = phy[kym >= 0 ? kym : k];
-        const C psxp = psr[kxp >= 0 ? kxp : k], psxm = psr[kxm >= 0 ? kxm : k];
-        const C psyp = psr[kyp >= 0 ? kyp : k], psym = psr[kym >= 0 ? kym : k];
-        const C pszp = psr[kzp >= 0 ? kzp : k];
-        const int64_t kn = last ? k : k + wxy;  // the next plane of this item
-        const C fxo_n = phx[kn], fyo_n = phy[kn], fzo_n = phz[kn];
-        const C zz = zcz[z], hz = zhz[z];
-        // ---- phi^n at the cell and at its -x, -y neighbours (phi_at's g2, g3)
-        const C fx = phi_update<C>(fxo, pc, xp, pm, pmxp, ps, kxp >= 0 ? psxp : (C)0, hx, zy + zz - hx, zy * zz,
-                                   c.cc, c.idt, c.idx);
-        const C fy = phi_update<C>(fyo, pc, yp, pm, pmyp, ps, kyp >= 0 ? psyp : (C)0, hy, zx + zz - hy, zx * zz,
-                                   c.cc, c.idt, c.idx);
-        const C fz = phi_update<C>(fzo, pc, u, pm, pmu, ps, kzp >= 0 ? pszp : (C)0, hz, zx + zy - hz, zx * zy,
-                                   c.cc, c.idt, c.idx);
-        const C fxm = kxm < 0 ? (C)0
-                              : phi_update<C>(fxom, xm, pc, pmxm, pm, psxm, ps, hxm, zy + zz - hxm, zy * zz, c.cc,
-                                              c.idt, c.idx);
-        const C fym = kym < 0 ? (C)0
-                              : phi_update<C>(fyom, ym, pc, pmym, pm, psym, ps, hym, zx + zz - hym, zx * zz, c.cc,
-                                              c.idt, c.idx);
-        C fzm = fzc;
-        if (dz == 0) {  // uniform: the plane below belongs to another item or box
-            const int64_t kzm = z - 1 >= bz0 ? k - wxy : pml_find(c.pml_boxes, r, d.x, x, y, z - 1);
-            const C fzom = phz[kzm >= 0 ? kzm : k], pszm = psr[kzm >= 0 ? kzm : k];
-            const C hzm = zhz[z - 1];
-            fzm = kzm < 0 ? (C)0
-                          : phi_update<C>(fzom, pd, pc, pmd, pm, pszm, ps, hzm, zx + zy - hzm, zx * zy, c.cc, c.idt,
-                                          c.idx);
-        }
-        const C psn = ps + c.pml.dt * pc;
-        // ---- stores
+        const int z = oz + dz;
+        C* L = lds + (dz & 1) * 2 * NT;
+        C fx = (C)0, fy = (C)0, fz = (C)0, pc = (C)0, ps = (C)0, psn = (C)0;
         if (active) {
+            const int64_t kxp = x + 1 < B[2] ? k + 1 : pml_find(c.pml_boxes, r, bd.x, x + 1, y, z);
+            const int64_t kyp = y + 1 < B[4] ? k + wx : pml_find(c.pml_boxes, r, bd.x, x, y + 1, z);
+            const int64_t kzp = z + 1 < B[6] ? k + wxy : pml_find(c.pml_boxes, r, bd.x, x, y, z + 1);
+            fx = phi_at<S, C>(c, r, 0, i, x, y, z, k, kxp, phx, psr, 1);
+            fy = phi_at<S, C>(c, r, 1, i, x, y, z, k, kyp, phy, psr, sy);
+            fz = phi_at<S, C>(c, r, 2, i, x, y, z, k, kzp, phz, psr, sz);
             c.phib[par][0][k] = fx;
             c.phib[par][1][k] = fy;
             c.phib[par][2][k] = fz;
+            pc = ld<S, C>(c.cur, i);
+            ps = psr[k];
+            psn = ps + c.pml.dt * pc;
             c.psib[par][k] = psn;
-            if (!((skip >> dz) & 1u)) {
-                const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
-                C sum = xp + xm;
-                sum = sum + yp;
-                sum = sum + ym;
-                sum = sum + u;
-                sum = sum + pd;
-                st<S, C>(c.out, i, pml_formula<C>(sum, pc, pm, dv, ps, psn, zx, zy, zz, c.pml.dt, c.pml.dx, c.k.l2));
-            }
+            L[tid] = fx;
+            L[NT + tid] = fy;
         }
-        // ---- next plane
+        __syncthreads();
+        if (active && !((skip >> dz) & 1u)) {
+            C fxm, fym, fzm;
+            if (tx > 0) {
+                fxm = L[tid - 1];
+            } else {
+                const int64_t kxm = x - 1 >= B[1] ? k - 1 : pml_find(c.pml_boxes, r, bd.x, x - 1, y, z);
+                fxm = phi_at<S, C>(c, r, 0, i - 1, x - 1, y, z, kxm, k, phx, psr, 1);
+            }
+            if (ty > 0) {
+                fym = L[NT + tid - ex];
+            } else {
+                const int64_t kym = y - 1 >= B[3] ? k - wx : pml_find(c.pml_boxes, r, bd.x, x, y - 1, z);
+                fym = phi_at<S, C>(c, r, 1, i - sy, x, y - 1, z, kym, k, phy, psr, sy);
+            }
+            if (dz > 0) {
+                fzm = fzc;
+            } else {
+                const int64_t kzm = z - 1 >= B[5] ? k - wxy : pml_find(c.pml_boxes, r, bd.x, x, y, z - 1);
+                fzm = phi_at<S, C>(c, r, 2, i - sz, x, y, z - 1, kzm, k, phz, psr, sz);
+            }
+            const C dv = (fx - fxm) + (fy - fym) + (fz - fzm);
+            C sum = ld<S, C>(c.cur, i + 1) + ld<S, C>(c.cur, i - 1);
+            sum = sum + ld<S, C>(c.cur, i + sy);
+            sum = sum + ld<S, C>(c.cur, i - sy);
+            sum = sum + ld<S, C>(c.cur, i + sz);
+            sum = sum + ld<S, C>(c.cur, i - sz);
+            const C v = pml_formula<C>(sum, pc, ld<S, C>(c.prev, i), dv, ps, psn, c.pml.z[0][r.axis_off[0] + x],
+                                       c.pml.z[1][r.axis_off[1] + y], c.pml.z[2][r.axis_off[2] + z], c.pml.dt,
+                                       c.pml.dx, c.k.l2);
+            st<S, C>(c.out, i, v);
+        }
         fzc = fz;
-        pd = pc;
-        pc = u;
-        pmd = pm;
-        pm = pmu;
-        fxo = fxo_n;
-        fyo = fyo_n;
-        fzo = fzo_n;
-        ps = pszp;
         k += wxy;
         i += sz;
     }
@@ -1266,7 +1229,7 @@ __global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Comput
     // origin packed in b (x | y << 11 | z << 22), room w; 4: the same tile
     // advanced two steps (interior_pair); 1: packed boundary entries
     // e_pack[a..b); 2: PML list [a..b); 3: special cells [a..b); 5: PML
-    // strip a (pml_strip); 6: PML list [a..b) of source/receiver cells
+    // block a (pml_block); 6: PML list [a..b) of source/receiver cells
     // whose phi and psi a PML block stores; -1: idle.
     // Items of one XCD (blockIdx % 8) follow its tiles in order, each list
     // chunk right before the tile holding its first cell, so both read the
@@ -1296,8 +1259,8 @@ __global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Comput
         pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, reinterpret_cast<C(*)[NT]>(lds));
         return;
     }
-    if (wk.x == 5) {  // PML strip: one column per thread, no barrier
-        pml_strip<S, C, NT>(c, wk.y, t);
+    if (wk.x == 5) {  // whole workgroup: barriers every plane
+        pml_block<S, C, NT>(c, wk.y, t, lds);
         return;
     }
     if (wk.x == 6) {  // PML source/receiver cells next to PML blocks, which store their phi and psi

@@ -63,10 +63,9 @@ extern "C" {
 #define AFX_SPLIT 4u        /* separate interior and boundary launches (no fused step kernel) */
 #define AFX_LIST_BLOCKS 8u  /* fused kernel: boundary and PML cells in workgroups of their own instead of
                                in the tile workgroups that hold them (same results) */
-#define AFX_PML_BLOCKS 16u  /* fused kernel: physical PML stepped by z-marching strips of PML columns
-                               (one thread per column, two planes, no barrier) instead of one thread
-                               per PML cell record (same results; 2 % faster in a 98 M-cell room,
-                               5 % slower in a 6 M-cell one, so off by default) */
+#define AFX_PML_BLOCKS 16u  /* fused kernel: physical PML stepped by z-marching blocks of PML columns
+                               instead of one thread per PML cell record (same results; slower on
+                               MI355X, whose latency the per-cell records hide better) */
 
 /* PML faces */
 #define AFX_FACE_XM 1u

@@ -1,4 +1,4 @@
-"""The physical PML stepped by z-marching PML strips (AFX_PML_BLOCKS)
+"""The physical PML stepped by z-marching PML blocks (AFX_PML_BLOCKS)
 against the per-cell PML records (the fused kernel's default) and the oracle,
 bit for bit: PML on several faces (boxes meeting at edges and corners),
 boundary cells and furniture inside the layer, sources and receivers in it,
@@ -55,7 +55,7 @@ def _run(v, a, b, sc, src, mc, steps, precision, width, faces, prof, tile=None, 
     ("fp32", None, 1), ("fp64", None, 1), ("fp16", None, 1), ("bf16", (128, 4, 8), 1),
     ("fp32", (256, 2, 8), 1), ("fp32", (64, 4, 16), 2), ("fp32", (256, 1, 16), 1)])
 def test_pml_blocks_match_records_and_oracle(precision, tile, tblock):
-    _case(f"PML strips, {precision}, tile {tile}, {tblock} step(s) per pass")
+    _case(f"PML blocks, {precision}, tile {tile}, {tblock} step(s) per pass")
     faces = 1 | 2 | 8 | 32
     v, a, b, rng = _room(91, faces=faces)
     width = 6
@@ -82,7 +82,7 @@ def test_pml_blocks_match_records_and_oracle(precision, tile, tblock):
 
 
 def test_pml_blocks_batched_rooms_match_single_runs():
-    _case("PML strips, 3 batched rooms, PML on x+ and z-")
+    _case("PML blocks, 3 batched rooms, PML on x+ and z-")
     ws = rooms.batch_rooms(seed=19, nrooms=3, steps=150, lo=(30, 24, 22), hi=(70, 40, 36), nmic=4)
     from anc_field_py import batch
     prof = (3e4 * ((np.arange(5) + 0.5) / 5) ** 3).astype(np.float32)
