@@ -1210,8 +1210,19 @@ __global__ __launch_bounds__(kBndThreads) void k_boundary(BndCtx<S, C> c, int nb
 //   [tile0, tile0+ntiles)  interior tiles; tile0 is a multiple of 8 so the
 //                          XCD-swizzled tile order keeps its XCD mapping
 // The latency-bound lists come first so they overlap the interior stream.
+// Minimum waves per SIMD the fused kernel is compiled for.  With 16-bit
+// storage and several cells per lane the register budget of seven waves (at
+// most 72 VGPRs) fits every work kind without spilling, and the 640x480x320
+// FP16 step runs 3.7 % faster (280.8 against 291.5 us).  Elsewhere the
+// compiler's own choice (74 VGPRs for FP32, 102 for FP64) is kept: the same
+// cap spills with one cell per lane and is slower for FP32.
+template <typename S, int V> struct FusedMinWaves {
+    static constexpr int value = (sizeof(S) == 2 && V > 1) ? 7 : 1;
+};
+
 template <typename S, int TX, int TY, int TZ, int V>
-__global__ __launch_bounds__(TX / V * TY) void k_fused(BndCtx<S, typename Compute<S>::T> c,
+__global__ __launch_bounds__(TX / V * TY) __attribute__((amdgpu_waves_per_eu(FusedMinWaves<S, V>::value)))
+void k_fused(BndCtx<S, typename Compute<S>::T> c,
                                                    const int4* __restrict__ tiles,
                                                    const uint32_t* __restrict__ colmask,
                                                    const int64_t* __restrict__ d_t, int t_rel,
