@@ -281,6 +281,17 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
     }
 }
 
+// Logical output tile of this workgroup.  Workgroup b runs on XCD b % 8;
+// when the grid is a multiple of 8, XCD x takes the contiguous run of tiles
+// [x * nb/8, (x+1) * nb/8), in order, so the tiles of one row panel of A
+// (consecutive in the row-major tile order) run together on one XCD and
+// read that panel through its L2 once, instead of once per XCD.
+__device__ __forceinline__ int xcd_tile()
+{
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // Inference GEMM for large M (HIDDEN and OUT epilogues): a 64x64 output tile
 // per workgroup, operands staged through LDS in k-steps of 32 (double
 // buffered, one barrier per step), each wave a 32x32 quadrant.  It gives the
@@ -295,8 +306,9 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
     __shared__ bf16 As[2][64][kPad];
     __shared__ bf16 Bs[2][64][kPad];
     const int tiles_n = g.N >> 6;
-    const int m0 = (int)(blockIdx.x / tiles_n) << 6;
-    const int n0 = (int)(blockIdx.x % tiles_n) << 6;
+    const int tile = xcd_tile();
+    const int m0 = (tile / tiles_n) << 6;
+    const int n0 = (tile % tiles_n) << 6;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave >> 1, wj = wave & 1;
     const int lr = tid >> 2, lc = (tid & 3) << 3;  // this thread's 16-byte piece of each k-step
@@ -373,8 +385,9 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
     __shared__ bf16 As[2][2][128][kPad];  // [buffer][k-step of the pair][row][k]
     __shared__ bf16 Bs[2][2][128][kPad];
     const int tiles_n = g.N >> 7;
-    const int m0 = (int)(blockIdx.x / tiles_n) << 7;
-    const int n0 = (int)(blockIdx.x % tiles_n) << 7;
+    const int tile = xcd_tile();
+    const int m0 = (tile / tiles_n) << 7;
+    const int n0 = (tile % tiles_n) << 7;
     const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave >> 2, wj = wave & 3;
     const int lr = tid >> 2, lc = (tid & 3) << 3;
