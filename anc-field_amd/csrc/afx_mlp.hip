@@ -17,7 +17,7 @@
 //                      OUT     y = C + b (FP32)
 //                      LOSS    y = C + b; g = 2 w (y - t) -> BF16 both layouts; per-wave loss partial
 //                      BACK    g = C * silu'(z_prev) -> BF16 both layouts
-//   k_gemm128<EPI>   HIDDEN and OUT for inference chunks (K a multiple of 256):
+//   k_gemm128<EPI>   HIDDEN and OUT for inference chunks (K a multiple of 128):
 //                    128x128 tiles through LDS, k-slices visited in turn (same
 //                    bits as k_gemm)
 //   k_gemm64<EPI>    HIDDEN and OUT for inference chunks: 64x64 tiles through LDS,
@@ -368,43 +368,82 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
 }
 
 // Inference GEMM for large M on 128x128 output tiles (HIDDEN and OUT
-// epilogues): eight waves, each a 64x32 part of 4x2 MFMA blocks, operands
-// staged through LDS two k-steps of 32 at a time (double buffered, 80 KiB,
-// one barrier per pair); 110 VGPRs, so four waves per SIMD hide the operand
-// loads.  The bits are k_gemm's: every MFMA sees the same operands per lane,
-// and each output is the sum of four k-slice partials (slice = k-step mod 4)
-// in k_gemm's order.  The k-steps are visited slice by slice (0, 4, 8, ...,
-// then 1, 5, 9, ...), so only the current slice's partials and the running
-// sum of the finished slices are held: ((p0 + p1) + p2) + p3 as in k_gemm64,
-// with half its accumulator registers per output.  Needs M and N multiples
-// of 128 and K of 256 (a pair of k-steps never straddles two slices).
+// epilogues): eight waves, each a 64x32 part of 4x2 MFMA blocks.  Operands
+// go straight from memory into LDS (global_load_lds_dwordx4: 16 B per lane,
+// no VGPRs), one k-step of 32 per stage, four stages in a ring: the loads
+// of k-step v + 2 are issued while k-step v is computed, one barrier per
+// k-step (a stage is refilled two k-steps after its last read, across a
+// barrier).  Six stages (one workgroup per CU) measured 40 % slower.  LDS rows are 64 B, with the 16-byte chunk c of
+// row r stored at chunk c ^ ((r >> 1) & 3): the fragment reads (16 rows per
+// quarter wave) and the loads' LDS writes meet no bank conflicts.  The bits
+// are k_gemm's: every MFMA sees the same operands per lane, and each output
+// is the sum of four k-slice partials (slice = k-step mod 4) in k_gemm's
+// order.  The k-steps are visited slice by slice (0, 4, 8, ..., then 1, 5,
+// 9, ...), so only the current slice's partials and the running sum of the
+// finished slices are held: ((p0 + p1) + p2) + p3 as in k_gemm64.  Needs M
+// and N multiples of 128 and K of 128.
+constexpr int kG128Stages = 4;  // 64 KiB of LDS: two workgroups per CU
+
+// s_waitcnt with vmcnt(n) and the other counters left open (gfx9 encoding)
+#define AFX_VM_WAIT(n) __builtin_amdgcn_s_waitcnt(0x0F70 | ((n) & 15) | ((((n) >> 4) & 3) << 14))
+
+// LDS position (in bf16) of 16-byte chunk c of operand row `row`
+__device__ __forceinline__ int g128_at(int row, int c) { return row * 32 + ((c ^ ((row >> 1) & 3)) << 3); }
+
+// Six 16-byte fragment reads from LDS (four of A at byte offsets 0, 1024,
+// 2048 and 3072 from pa, two of B at 0 and 1024 from pb), with the wait for
+// them in the same block, so every output holds its value when the block
+// ends.  Written out because the compiler, seeing LDS reads while loads into
+// LDS are in flight, would wait for all of them first.
+__device__ __forceinline__ void g128_frags(uint32_t pa, uint32_t pb, bf16x8 (&a)[4], bf16x8 (&b)[2])
+{
+    asm volatile(
+        "ds_read_b128 %0, %6\n\t"
+        "ds_read_b128 %1, %6 offset:1024\n\t"
+        "ds_read_b128 %2, %6 offset:2048\n\t"
+        "ds_read_b128 %3, %6 offset:3072\n\t"
+        "ds_read_b128 %4, %7\n\t"
+        "ds_read_b128 %5, %7 offset:1024\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(a[0]), "=&v"(a[1]), "=&v"(a[2]), "=&v"(a[3]), "=&v"(b[0]), "=&v"(b[1])
+        : "v"(pa), "v"(pb)
+        : "memory");
+}
+
 template <int E>
 __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
 {
-    constexpr int kPad = 40;  // LDS row stride in bf16 (32 + 8)
-    __shared__ bf16 As[2][2][128][kPad];  // [buffer][k-step of the pair][row][k]
-    __shared__ bf16 Bs[2][2][128][kPad];
+    constexpr int S = kG128Stages;
+    static_assert(S >= 3 && 2 * (S - 2) < 64, "stages");
+    __shared__ bf16 As[S][4096];  // [stage][row * 32 + k], chunks swizzled (g128_at)
+    __shared__ bf16 Bs[S][4096];
     const int tiles_n = g.N >> 7;
     const int tile = xcd_tile();
     const int m0 = (tile / tiles_n) << 7;
     const int n0 = (tile % tiles_n) << 7;
-    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wi = wave >> 2, wj = wave & 3;
-    const int lr = tid >> 2, lc = (tid & 3) << 3;
-    const bf16* ga = g.A + (size_t)(m0 + lr) * g.lda + lc;
-    const bf16* gb = g.B + (size_t)(n0 + lr) * g.ldb + lc;
-    const int nk = g.K >> 5, ns = nk >> 2;  // ns even: a pair never straddles two slices
+    const int nk = g.K >> 5, ns = nk >> 2;
     auto kof = [&](int v) { return 32 * (v / ns + 4 * (v % ns)); };
-    bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(ga + kof(0));
-    bf16x8 ra1 = *reinterpret_cast<const bf16x8*>(ga + kof(1));
-    bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(gb + kof(0));
-    bf16x8 rb1 = *reinterpret_cast<const bf16x8*>(gb + kof(1));
-    *reinterpret_cast<bf16x8*>(&As[0][0][lr][lc]) = ra0;
-    *reinterpret_cast<bf16x8*>(&As[0][1][lr][lc]) = ra1;
-    *reinterpret_cast<bf16x8*>(&Bs[0][0][lr][lc]) = rb0;
-    *reinterpret_cast<bf16x8*>(&Bs[0][1][lr][lc]) = rb1;
-    __syncthreads();
-    const int r = lane & 15, kq = (lane >> 4) << 3;
+    // this lane's 16-byte chunk of a stage: slot q = tid holds row q >> 2,
+    // stored chunk q & 3, i.e. logical chunk (q & 3) ^ ((row >> 1) & 3)
+    const int lrow = tid >> 2;
+    const int lchunk = (tid & 3) ^ ((lrow >> 1) & 3);
+    const bf16* ga = g.A + (size_t)(m0 + lrow) * g.lda + 8 * lchunk;
+    const bf16* gb = g.B + (size_t)(n0 + lrow) * g.ldb + 8 * lchunk;
+    auto issue = [&](int v) {  // k-step v into stage v % S; past the end: the last k-step again
+        const int k = kof(v < nk ? v : nk - 1);
+        __builtin_amdgcn_global_load_lds(ga + k, &As[v % S][0] + wave * 512, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(gb + k, &Bs[v % S][0] + wave * 512, 16, 0, 0);
+    };
+#pragma unroll
+    for (int v = 0; v < S - 2; ++v) issue(v);
+    const int r = lane & 15, kc = lane >> 4;
+    // fragment rows 64 wi + 16 i + r (A) and 32 wj + 16 j + r (B) share the
+    // chunk swizzle ((row >> 1) & 3 does not see the 16 i), so the four A
+    // (two B) fragments sit 1024 bytes apart
+    const uint32_t oa = 2u * (uint32_t)g128_at(64 * wi + r, kc), ob = 2u * (uint32_t)g128_at(32 * wj + r, kc);
     f32x4 sum[4][2], p[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -413,30 +452,21 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
             sum[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
-    for (int v = 0; v < nk; v += 2) {
-        const int cur = (v >> 1) & 1;
-        const bool more = v + 2 < nk;
-        if (more) {
-            ra0 = *reinterpret_cast<const bf16x8*>(ga + kof(v + 2));
-            ra1 = *reinterpret_cast<const bf16x8*>(ga + kof(v + 3));
-            rb0 = *reinterpret_cast<const bf16x8*>(gb + kof(v + 2));
-            rb1 = *reinterpret_cast<const bf16x8*>(gb + kof(v + 3));
-        }
+    for (int v = 0; v < nk; ++v) {
+        // two loads per wave per k-step, retired in order: once at most
+        // 2 (S - 2) are outstanding, k-step v is in LDS
+        issue(v + S - 2);
+        AFX_VM_WAIT(2 * (S - 2));
+        __builtin_amdgcn_s_barrier();
+        bf16x8 a[4], b[2];
+        g128_frags((uint32_t)(uintptr_t)&As[v % S][0] + oa, (uint32_t)(uintptr_t)&Bs[v % S][0] + ob, a, b);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            bf16x8 a[4], b[2];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][h][64 * wi + 16 * i + r][kq]);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][h][32 * wj + 16 * j + r][kq]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
-        }
-        if ((v + 1) % ns == ns - 1) {  // slice (v + 1) / ns complete
-            const bool first = v + 1 < ns;
+            for (int j = 0; j < 2; ++j)
+                p[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], p[i][j], 0, 0, 0);
+        if (v % ns == ns - 1) {  // slice v / ns complete
+            const bool first = v < ns;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -445,14 +475,8 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
                     p[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                 }
         }
-        if (more) {
-            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][0][lr][lc]) = ra0;
-            *reinterpret_cast<bf16x8*>(&As[cur ^ 1][1][lr][lc]) = ra1;
-            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][0][lr][lc]) = rb0;
-            *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][1][lr][lc]) = rb1;
-        }
-        __syncthreads();
     }
+    AFX_VM_WAIT(0);  // no load into LDS outlives the workgroup
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -732,7 +756,7 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
     // large-M inference GEMMs on 128x128 or 64x64 tiles (same bits; see
     // k_gemm128 and k_gemm64)
     if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 128 == 0 && g.N % 128 == 0 &&
-        g.K % 256 == 0) {
+        g.K % 128 == 0) {
         const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(512);
         HIP_CHECK(afx::launch_ok(grid128, block128, 512));
         if (epi == EPI_HIDDEN)
