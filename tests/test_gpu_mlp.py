@@ -181,3 +181,25 @@ def test_mlp_large_m_gemm_matches_training_kernels():
     assert np.array_equal(y_128[:256], y_train)
     assert np.array_equal(y_64[:256], y_train)
     assert np.array_equal(y_64, y_128[:1088])
+
+
+@pytest.mark.parametrize("in_dim,n_freq,width,out_dim", [(3, 64, 384, 128), (2, 32, 128, 256)])
+def test_mlp_large_m_gemm_other_shapes_match_training_kernels(in_dim, n_freq, width, out_dim):
+    """The 128x128 inference tiles on shapes off the surrogate's own: K of
+    384 (a multiple of 128, not of 256) and of 128 (one k-step per slice),
+    and 1152 rows, whose 9 x 3 (or 9 x 1, 9 x 2) tiles are not a multiple of
+    8, so the tiles keep the row-major order instead of the XCD-grouped one.
+    Outputs equal the training path's split-K kernels bit for bit."""
+    print(f"\n[case] mlp inference tiles, in_dim {in_dim}, n_freq {n_freq}, width {width}, out {out_dim}", flush=True)
+    from anc_field_py import _afx
+    rng = np.random.default_rng(11)
+    rows = 1152
+    x = rng.uniform(0, 1, (rows, in_dim)).astype(np.float32)
+    t = (0.05 * rng.standard_normal((256, out_dim))).astype(np.float32)
+    w = np.full(256, 1.0 / 256, np.float32)
+    with _afx.Mlp(seed=22, in_dim=in_dim, n_freq=n_freq, width=width, out_dim=out_dim, depth=4) as m:
+        assert 2 * in_dim * n_freq in (384, 128)
+        y_train = np.concatenate([m.gradients(x[k:k + 256], t, w)[2] for k in range(0, 1024, 256)])
+        y = m.forward(x)  # 1152 rows: 128x128 tiles
+    assert np.abs(y).max() > 0
+    assert np.array_equal(y[:1024], y_train)
