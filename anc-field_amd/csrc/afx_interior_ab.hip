@@ -1,7 +1,9 @@
 // Interior-update variants measured side by side (afx-interior-ab): the
 // leapfrog update of every interior cell of an all-air grid, the operation of
-// the library's interior tiles, written three ways, plus a plain copy of the
-// same bytes.  Each variant's output is compared bit for bit with variant A.
+// the library's interior tiles, written several ways, plus a plain copy of
+// the same bytes (half read, half written) and a stream with a step's byte
+// mix (two fields read, one written, no neighbours).  Each variant's output is
+// compared bit for bit with variant A.
 //   A  col: one column per thread, z-march with four planes of p^n in flight,
 //      x and y neighbours from the L1 / L2 (the library's interior_column);
 //   B  vec: V cells per lane along x, x neighbours across lanes by shuffles
@@ -474,6 +476,16 @@ __global__ void k_copy(const float4* __restrict__ a, float4* __restrict__ b, lon
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
+// The step's own byte mix with no stencil: two fields read, one written, 16 B
+// per lane (out = a - b, bitwise-meaningless for FP16 but the same bytes).
+__global__ void k_stream(const float4* __restrict__ a, const float4* __restrict__ b, float4* __restrict__ c, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float4 u = a[i], v = b[i];
+        c[i] = make_float4(u.x - v.x, u.y - v.y, u.z - v.z, u.w - v.w);
+    }
+}
+
 template <typename S>
 void run(Room r, const char* tag)
 {
@@ -593,6 +605,12 @@ void run(Room r, const char* tag)
         auto L = [&]() { hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, 0, (const float4*)src, (float4*)dst, n16); };
         timeit("copy (same bytes)", L, dst, false);
         CK(hipFree(src)); CK(hipFree(dst));
+    }
+    {
+        // p^n and p^{n-1} read, p^{n+1} written, no neighbours: the byte mix of a step
+        const long n16 = sizeof(S) * n / 16;
+        auto L = [&]() { hipLaunchKernelGGL(k_stream, dim3(4096), dim3(256), 0, 0, (const float4*)cur, (const float4*)prev, (float4*)outB, n16); };
+        timeit("stream 2 read 1 write", L, outB, false);
     }
     CK(hipFree(cur)); CK(hipFree(prev)); CK(hipFree(outA)); CK(hipFree(outB));
 }
