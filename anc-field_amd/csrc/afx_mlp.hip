@@ -148,11 +148,17 @@ struct AdamTable {
     int n;
 };
 
-__device__ inline float silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU and its derivative from the hardware exponential and reciprocal
+// (v_exp_f32 on z log2 e, v_rcp_f32): relative error about 1e-6, far below
+// the BF16 rounding of the stored activation (2^-8), and a few instructions
+// instead of a libm exponential and an IEEE division.  Training and inference
+// call the same functions, so their activations keep equal bits.
+__device__ inline float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid_fast(z); }
 
 __device__ inline float silu_grad(float z)
 {
-    const float s = 1.0f / (1.0f + expf(-z));
+    const float s = sigmoid_fast(z);
     return s + z * s * (1.0f - s);
 }
 
@@ -246,18 +252,20 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
     const int mb = m0 + 16 * i + ((lane >> 4) << 2);
     float loss = 0.0f;
     bf16x4 col;
+    // loaded once: the stores below may alias it as far as the compiler knows
+    const float bn = E == EPI_BACK ? 0.0f : g.bias[n];
     for (int e = 0; e < 4; ++e) {
         const int m = mb + e;
         float v = acc[e];
         if (E == EPI_HIDDEN) {
-            const float z = v + g.bias[n];
+            const float z = v + bn;
             if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
             v = silu(z);
         } else if (E == EPI_OUT) {
-            g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+            g.f32[(size_t)m * g.ldf + n] = v + bn;
             continue;
         } else if (E == EPI_LOSS) {
-            const float y = v + g.bias[n];
+            const float y = v + bn;
             if (g.f32) g.f32[(size_t)m * g.ldf + n] = y;
             const int src = g.rows ? g.rows[step_of(g.sr) * g.sr.rows_stride + m] : m;
             const float w = g.wrow[src];
@@ -349,6 +357,11 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
             __syncthreads();
         }
     }
+    // biases loaded before any store; 32-bit offsets (the host checks
+    // M x ld < 2^31 for this path)
+    float bj[2];
+    for (int j = 0; j < 2; ++j) bj[j] = g.bias[n0 + 32 * wj + 16 * j + r];
+    const bool keep = g.f32 != nullptr;
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) {
             const int n = n0 + 32 * wj + 16 * j + r;
@@ -357,11 +370,11 @@ __global__ __launch_bounds__(256) void k_gemm64(Gemm g)
                 const float v = ((p[0][i][j][e] + p[1][i][j][e]) + p[2][i][j][e]) + p[3][i][j][e];
                 const int m = mb + e;
                 if (E == EPI_HIDDEN) {
-                    const float z = v + g.bias[n];
-                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
-                    g.act[(size_t)m * g.ldact + n] = (bf16)silu(z);
+                    const float z = v + bj[j];
+                    if (keep) g.f32[m * g.ldf + n] = z;
+                    g.act[m * g.ldact + n] = (bf16)silu(z);
                 } else {
-                    g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+                    g.f32[m * g.ldf + n] = v + bj[j];
                 }
             }
         }
@@ -477,6 +490,12 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
         }
     }
     AFX_VM_WAIT(0);  // no load into LDS outlives the workgroup
+    // biases loaded before any store (which the compiler would otherwise
+    // assume may change them); 32-bit offsets (the host checks M x ld < 2^31)
+    float bj[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bj[j] = g.bias[n0 + 32 * wj + 16 * j + r];
+    const bool keep = g.f32 != nullptr;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -488,11 +507,11 @@ __global__ __launch_bounds__(512) void k_gemm128(Gemm g)
                 const float v = sum[i][j][e];
                 const int m = mb + e;
                 if (E == EPI_HIDDEN) {
-                    const float z = v + g.bias[n];
-                    if (g.f32) g.f32[(size_t)m * g.ldf + n] = z;
-                    g.act[(size_t)m * g.ldact + n] = (bf16)silu(z);
+                    const float z = v + bj[j];
+                    if (keep) g.f32[m * g.ldf + n] = z;
+                    g.act[m * g.ldact + n] = (bf16)silu(z);
                 } else {
-                    g.f32[(size_t)m * g.ldf + n] = v + g.bias[n];
+                    g.f32[m * g.ldf + n] = v + bj[j];
                 }
             }
         }
@@ -531,26 +550,46 @@ __device__ inline void grads_tile(const GradTable& t, AdamArgs ad, const StepRef
     *reinterpret_cast<bf16x4*>(g.wt + (size_t)n * g.M + mb) = col;
 }
 
-// Fourier features: a block holds blockDim.z rows; thread (k, j, r) writes
-// sin and cos of 2 pi f_k x[b][j], b = blockIdx.x * blockDim.z + r, at
-// features j*2F + k and j*2F + F + k, in X [rows][nfeat] and X^T [nfeat][ldt].
+// Fourier features: a block holds blockDim.z rows; thread (q, j, r) writes
+// sin and cos of 2 pi f_k x[b][j] for the KV frequencies k = KV q .. KV q +
+// KV - 1, b = blockIdx.x * blockDim.z + r, at features j*2F + k and
+// j*2F + F + k, in X [rows][nfeat] (one KV-element store each; KV divides
+// F) and X^T [nfeat][ldt].
+template <int KV> struct alignas(2 * KV) Bf16Pack {
+    bf16 v[KV];
+};
+
+template <int KV>
 __global__ void k_features(const float* __restrict__ x, const int* __restrict__ rows, int nrows, int in_dim, int nf,
                            const float* __restrict__ freq, bf16* __restrict__ X, bf16* __restrict__ XT, int ldt,
                            StepRef sr)
 {
-    const int k = (int)threadIdx.x, j = (int)threadIdx.y, b = (int)(blockIdx.x * blockDim.z + threadIdx.z);
-    if (b >= nrows || k >= nf || j >= in_dim) return;
+    const int k0 = KV * (int)threadIdx.x, j = (int)threadIdx.y, b = (int)(blockIdx.x * blockDim.z + threadIdx.z);
+    if (b >= nrows || k0 >= nf || j >= in_dim) return;
     const int src = rows ? rows[step_of(sr) * sr.rows_stride + b] : b;
-    const float cyc = freq[k] * x[(size_t)src * in_dim + j];
-    float s, c;
-    sincospif(2.0f * cyc, &s, &c);
+    const float xj = x[(size_t)src * in_dim + j];
+    Bf16Pack<KV> sv, cv;
+#pragma unroll
+    for (int u = 0; u < KV; ++u) {
+        const float cyc = freq[k0 + u] * xj;
+        // v_sin_f32 / v_cos_f32 take their argument in turns; cyc less its
+        // nearest integer (exact in FP32) is the same angle within half a
+        // turn, where they are accurate to about 1e-6, far below the BF16
+        // rounding
+        const float ph = cyc - rintf(cyc);
+        sv.v[u] = (bf16)__builtin_amdgcn_sinf(ph);
+        cv.v[u] = (bf16)__builtin_amdgcn_cosf(ph);
+    }
     const int nfeat = 2 * in_dim * nf;
-    const int fs = j * 2 * nf + k, fc = fs + nf;
-    X[(size_t)b * nfeat + fs] = (bf16)s;
-    X[(size_t)b * nfeat + fc] = (bf16)c;
+    const int fs = j * 2 * nf + k0, fc = fs + nf;
+    *reinterpret_cast<Bf16Pack<KV>*>(X + (size_t)b * nfeat + fs) = sv;
+    *reinterpret_cast<Bf16Pack<KV>*>(X + (size_t)b * nfeat + fc) = cv;
     if (XT) {
-        XT[(size_t)fs * ldt + b] = (bf16)s;
-        XT[(size_t)fc * ldt + b] = (bf16)c;
+#pragma unroll
+        for (int u = 0; u < KV; ++u) {
+            XT[(size_t)(fs + u) * ldt + b] = sv.v[u];
+            XT[(size_t)(fc + u) * ldt + b] = cv.v[u];
+        }
     }
 }
 
@@ -754,9 +793,10 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
 {
     if (g.M % 32 || g.N % 32 || g.K % 32) return err(AFX_E_ARG, "gemm %dx%dx%d not a multiple of 32", g.M, g.N, g.K);
     // large-M inference GEMMs on 128x128 or 64x64 tiles (same bits; see
-    // k_gemm128 and k_gemm64)
-    if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 128 == 0 && g.N % 128 == 0 &&
-        g.K % 128 == 0) {
+    // k_gemm128 and k_gemm64), whose epilogues index outputs in 32 bits
+    const bool off32 = (int64_t)g.M * std::max(g.ldf, g.ldact) < ((int64_t)1 << 31);
+    if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && off32 && g.M >= 1024 && g.M % 128 == 0 &&
+        g.N % 128 == 0 && g.K % 128 == 0) {
         const dim3 grid128((unsigned)((g.M / 128) * (g.N / 128))), block128(512);
         HIP_CHECK(afx::launch_ok(grid128, block128, 512));
         if (epi == EPI_HIDDEN)
@@ -766,8 +806,8 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
         HIP_CHECK(hipGetLastError());
         return AFX_OK;
     }
-    if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && g.M >= 1024 && g.M % 64 == 0 && g.N % 64 == 0 &&
-        g.K % 128 == 0) {
+    if ((epi == EPI_HIDDEN || epi == EPI_OUT) && !g.actT && off32 && g.M >= 1024 && g.M % 64 == 0 &&
+        g.N % 64 == 0 && g.K % 128 == 0) {
         const dim3 grid64((unsigned)((g.M / 64) * (g.N / 64))), block64(256);
         HIP_CHECK(afx::launch_ok(grid64, block64, 256));
         if (epi == EPI_HIDDEN)
@@ -795,14 +835,21 @@ int gemm_launch(afx_mlp* s, int epi, Gemm g)
 hipError_t launch_features(afx_mlp* s, const float* x, const int* rows, int nrows, bf16* X, bf16* XT, int ldt,
                            StepRef sr)
 {
-    // nf x in_dim threads per row (at most 1024, checked at creation), rows
-    // stacked in z up to about 256 threads per block
-    const int per_row = s->cfg.n_freq * s->cfg.in_dim;
+    // nf / KV x in_dim threads per row (at most 1024, checked at creation),
+    // rows stacked in z up to about 256 threads per block; four frequencies
+    // per thread (8-byte stores) when n_freq allows
+    const int kv = s->cfg.n_freq % 4 == 0 ? 4 : 1;
+    const int per_row = s->cfg.n_freq / kv * s->cfg.in_dim;
     const int rz = std::max(1, std::min(256 / per_row, 1024 / per_row));
-    const dim3 block((unsigned)s->cfg.n_freq, (unsigned)s->cfg.in_dim, (unsigned)rz);
-    if (hipError_t le = afx::launch_ok(dim3((unsigned)((nrows + rz - 1) / rz)), block, 1024)) return le;
-    k_features<<<dim3((unsigned)((nrows + rz - 1) / rz)), block, 0, s->stream>>>(x, rows, nrows, s->cfg.in_dim,
-                                                                               s->cfg.n_freq, s->freq, X, XT, ldt, sr);
+    const dim3 block((unsigned)(s->cfg.n_freq / kv), (unsigned)s->cfg.in_dim, (unsigned)rz);
+    const dim3 grid((unsigned)((nrows + rz - 1) / rz));
+    if (hipError_t le = afx::launch_ok(grid, block, 1024)) return le;
+    if (kv == 4)
+        k_features<4><<<grid, block, 0, s->stream>>>(x, rows, nrows, s->cfg.in_dim, s->cfg.n_freq, s->freq, X, XT, ldt,
+                                                     sr);
+    else
+        k_features<1><<<grid, block, 0, s->stream>>>(x, rows, nrows, s->cfg.in_dim, s->cfg.n_freq, s->freq, X, XT, ldt,
+                                                     sr);
     return hipGetLastError();
 }
 
