@@ -312,6 +312,67 @@ hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t st)
     return e;
 }
 
+// Evens out the work items of one kind (boundary or PML chunks) over the
+// eight XCD sequences of a step.  A chunk is dealt to the XCD of the tile
+// holding its first cell, so both read the same planes through one L2; but
+// walls along one tile band (the y walls without XCD rotation) then load one
+// or two XCDs with most of their kind while the others idle at the step's
+// end.  Sequences above the mean give up their surplus, chosen evenly along
+// the sequence, and those below it take them, spread evenly between their
+// own items.
+void balance_kind(std::vector<std::vector<int4>>& sq, int kind)
+{
+    const size_t nx = sq.size();
+    std::vector<size_t> cnt(nx, 0);
+    size_t total = 0;
+    for (size_t x = 0; x < nx; ++x) {
+        for (const int4& w : sq[x]) cnt[x] += w.x == kind ? 1 : 0;
+        total += cnt[x];
+    }
+    if (total == 0) return;
+    const size_t target = (total + nx - 1) / nx;
+    std::vector<int4> pool;
+    for (size_t x = 0; x < nx; ++x) {
+        if (cnt[x] <= target) continue;
+        const size_t excess = cnt[x] - target;
+        std::vector<int4> keep;
+        keep.reserve(sq[x].size());
+        size_t seen = 0;
+        for (const int4& w : sq[x]) {
+            if (w.x == kind) {
+                const bool take = (seen + 1) * excess / cnt[x] > seen * excess / cnt[x];
+                ++seen;
+                if (take) {
+                    pool.push_back(w);
+                    continue;
+                }
+            }
+            keep.push_back(w);
+        }
+        sq[x].swap(keep);
+        cnt[x] = target;
+    }
+    size_t pi = 0;
+    for (size_t y = 0; y < nx && pi < pool.size(); ++y) {
+        if (cnt[y] >= target) continue;
+        const size_t need = std::min(target - cnt[y], pool.size() - pi);
+        const size_t len = sq[y].size();
+        std::vector<int4> out;
+        out.reserve(len + need);
+        size_t ins = 0;
+        for (size_t k = 0; k <= len; ++k) {
+            while (ins < need && (ins + 1) * len / (need + 1) <= k) {
+                out.push_back(pool[pi++]);
+                ++ins;
+            }
+            if (k < len) out.push_back(sq[y][k]);
+        }
+        sq[y].swap(out);
+        cnt[y] += need;
+    }
+    for (size_t y = 0; pi < pool.size(); y = (y + 1) % nx) sq[y].push_back(pool[pi++]);
+}
+
 // Tile table in XCD-swizzled order.  Workgroup i runs on XCD i % 8.
 // Logical tiles (x fastest, then y, then z) are dealt to the XCDs in groups
 // of G consecutive tiles, G an eighth of room 0's z-layer of tiles, round
@@ -795,7 +856,9 @@ int build_special(afx_sim* s)
         }
         HIP_TRY(upload(s->d_e_pack, epack, s->stream));
         HIP_TRY(upload(s->d_pml_nbd, nbd, s->stream));
-        auto interleave = [](const std::vector<std::vector<int4>>& sq) {
+        auto interleave = [](std::vector<std::vector<int4>> sq) {
+            balance_kind(sq, 1);
+            balance_kind(sq, 2);
             size_t len = 0;
             for (const auto& v : sq) len = std::max(len, v.size());
             std::vector<int4> work(std::max<size_t>(8 * len, 1), make_int4(-1, 0, 0, 0));
