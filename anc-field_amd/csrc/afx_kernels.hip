@@ -1245,6 +1245,9 @@ void k_fused(BndCtx<S, typename Compute<S>::T> c,
     // Items of one XCD (blockIdx % 8) follow its tiles in order, each list
     // chunk right before the tile holding its first cell, so both read the
     // same planes of p^n through L2.
+    // the step index is read here, in the same round as the work item, not
+    // after it: list workgroups would otherwise wait on one more dependent load
+    const int64_t t = *d_t + t_rel;
     const int4 wk = work[blockIdx.x];
     if constexpr (kPair == 0) {
         if (wk.x == 4) return;  // no two-step tiles are dealt to these shapes
@@ -1265,7 +1268,6 @@ void k_fused(BndCtx<S, typename Compute<S>::T> c,
         return;
     }
     if (wk.x < 0) return;
-    const int64_t t = *d_t + t_rel;
     if (wk.x == 2) {  // whole workgroup: it meets a barrier
         pml_chunk<S, C, NT>(c, wk.y, wk.z, t, pml_nbd, reinterpret_cast<C(*)[NT]>(lds));
         return;
