@@ -28,6 +28,8 @@ DEVICE = [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the first 16 kernel-argument dwords arrive in SGPRs at wave start
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
 ]
 
 SOURCES = ["afx_kernels.hip", "afx_host.cpp", "afx_kerkythea.cpp", "afx_mlp.hip"]

@@ -1222,14 +1222,13 @@ template <typename S, int V> struct FusedMinWaves {
 
 template <typename S, int TX, int TY, int TZ, int V>
 __global__ __launch_bounds__(TX / V * TY) __attribute__((amdgpu_waves_per_eu(FusedMinWaves<S, V>::value)))
-void k_fused(BndCtx<S, typename Compute<S>::T> c,
-                                                   const int4* __restrict__ tiles,
-                                                   const uint32_t* __restrict__ colmask,
-                                                   const int64_t* __restrict__ d_t, int t_rel,
-                                                   const int4* __restrict__ work,
-                                                   const int4* __restrict__ e_pack,
-                                                   const uint32_t* __restrict__ pml_nbd)
+void k_fused(const int4* __restrict__ work, const int64_t* __restrict__ d_t, int t_rel,
+             const int4* __restrict__ e_pack, const uint32_t* __restrict__ pml_nbd,
+             const uint32_t* __restrict__ colmask, BndCtx<S, typename Compute<S>::T> c)
 {
+    // The small arguments come first: the build preloads the first kernel
+    // argument dwords into SGPRs, so the work item's address is known at wave
+    // start instead of after a load from the argument segment.
     using C = typename Compute<S>::T;
     constexpr int NT = TX / V * TY;   // threads
     constexpr int NCOL = TX * TY;     // columns (column-mask words per tile slot)
@@ -1610,8 +1609,8 @@ static hipError_t fused_t(const StepArgs& a, hipStream_t s)
     BndCtx<S, C> c = make_ctx<S, C>(a);
     if (a.work) {
         AFX_CHECK_LAUNCH(dim3(a.nwork), dim3(NT), NT);
-        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ, V>), dim3(a.nwork), dim3(NT), 0, s, c, a.tiles, a.colmask, a.d_t,
-                           a.t_rel, a.work, a.e_pack, a.pml_nbd);
+        hipLaunchKernelGGL((k_fused<S, TX, TY, TZ, V>), dim3(a.nwork), dim3(NT), 0, s, a.work, a.d_t, a.t_rel,
+                           a.e_pack, a.pml_nbd, a.colmask, c);
         return hipGetLastError();
     }
     const int nblk_b = (a.nb + NT - 1) / NT;
