@@ -98,6 +98,8 @@ _SIGS = {
     "afx_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "afx_set_time_block": (C.c_int, [_P, C.c_int32]),
     "afx_get_time_block": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "afx_set_tile_split": (C.c_int, [_P, C.c_int32]),
+    "afx_get_tile_split": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "afx_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -345,6 +347,16 @@ class Sim:
         """1: one step per pass; 2: two steps per pass where tiles allow it
         (bit-identical results)."""
         _check(lib.afx_set_time_block(self._h, steps))
+
+    def set_tile_split(self, split):
+        """0: whole tiles; 1 / 2: tiles in the top quarter / half of each
+        room's planes run as two halves (bit-identical results)."""
+        _check(lib.afx_set_tile_split(self._h, split))
+
+    def tile_split(self):
+        k = C.c_int32()
+        _check(lib.afx_get_tile_split(self._h, C.byref(k)))
+        return k.value
 
     def time_block(self):
         """(steps per pass, tiles taking two steps per pass)."""

@@ -175,6 +175,12 @@ struct afx_sim {
     // y-face walls and PML slabs over the XCDs); off, a band stays on one XCD
     // through z, so z-neighbour tiles share its L2.  The autotuner chooses.
     bool xcd_rotate = true;
+    // tiles whose origin lies in the top split/4 of their room's planes
+    // (0, 1 or 2: none, the top quarter, the top half) are dealt to the
+    // fused step as two work items of half the planes each: the last
+    // workgroups of a step are then shorter, and so is its tail.  The
+    // autotuner chooses.
+    int split = 0;
     bool fused_ok = true;
 
     // ---- device state
@@ -819,6 +825,15 @@ int build_special(afx_sim* s)
                     const int4 t4 = table[(size_t)sl];
                     if (t4.x < 0 || (mode == 2 && pair_ok[(size_t)sl])) continue;
                     const int kind = mode == 1 && pair_ok[(size_t)sl] ? 4 : 0;
+                    const Room& rr = s->rooms[(size_t)t4.x];
+                    if (kind == 0 && s->split > 0 && ts.tz >= 2 && 4 * t4.w >= (4 - s->split) * rr.nz) {
+                        // planes [0, tz/2) and [tz/2, tz) of the tile as two items
+                        const int h = ts.tz / 2;
+                        v.push_back(make_int4(0, (int)sl, (int)pack_xyz(t4.y, t4.z, t4.w), t4.x | (h << 24)));
+                        v.push_back(make_int4(0, (int)sl, (int)pack_xyz(t4.y, t4.z, t4.w),
+                                              t4.x | (h << 16) | (ts.tz << 24)));
+                        continue;
+                    }
                     v.push_back(make_int4(kind, (int)sl, (int)pack_xyz(t4.y, t4.z, t4.w), t4.x));
                 }
             }
@@ -2236,11 +2251,12 @@ int snap_restore(afx_sim* s, const Snapshot& sn)
 // Tile shape, steps per pass and XCD layer rotation of a committed handle:
 // the fourth field is allocated (zeroed) the first time two steps per pass
 // are chosen.
-int set_tile_tb(afx_sim* s, int tile, int tb, bool rotate)
+int set_tile_tb(afx_sim* s, int tile, int tb, bool rotate, int split)
 {
     s->tile = tile;
     s->tblock = tb;
     s->xcd_rotate = rotate;
+    s->split = split;
     s->drop_graphs();
     if (tb == 2 && !s->P[3].p) {
         HIP_TRY(s->P[3].alloc(s->P[0].bytes));
@@ -2251,7 +2267,7 @@ int set_tile_tb(afx_sim* s, int tile, int tb, bool rotate)
     return build_special(s);
 }
 
-int set_tile(afx_sim* s, int tile) { return set_tile_tb(s, tile, s->tblock, s->xcd_rotate); }
+int set_tile(afx_sim* s, int tile) { return set_tile_tb(s, tile, s->tblock, s->xcd_rotate, s->split); }
 
 bool can_pair(const afx_sim* s, int tile)
 {
@@ -2286,15 +2302,19 @@ struct TuneKey {
 struct TuneChoice {
     int tile, tblock;
     bool rotate;
-    bool operator!=(const TuneChoice& o) const { return tile != o.tile || tblock != o.tblock || rotate != o.rotate; }
+    int split;
+    bool operator!=(const TuneChoice& o) const
+    {
+        return tile != o.tile || tblock != o.tblock || rotate != o.rotate || split != o.split;
+    }
 };
 std::map<TuneKey, TuneChoice> g_tune_cache;
 std::mutex g_tune_mutex;
 
 // Tuning results also persist across processes in a text file, one line per
-// key:  <device> <prec> <compat> <pml> <flags> <nrooms> <nx ny nz ...> <tile> <tblock> <rotate>.
-// The file is AFX_TUNE_CACHE, else $XDG_CACHE_HOME/afx/tune-v1.txt, else
-// $HOME/.cache/afx/tune-v1.txt; AFX_TUNE_CACHE=off keeps results in the
+// key:  <device> <prec> <compat> <pml> <flags> <nrooms> <nx ny nz ...> <tile> <tblock> <rotate>
+// <split>.  The file is AFX_TUNE_CACHE, else $XDG_CACHE_HOME/afx/tune-v2.txt,
+// else $HOME/.cache/afx/tune-v2.txt; AFX_TUNE_CACHE=off keeps results in the
 // process only.  The device field is the architecture name and CU count, so
 // a file shared between machines keeps their results apart.  Lines that do
 // not parse, or name a tile shape this build lacks, are ignored.
@@ -2302,8 +2322,8 @@ std::string tune_file_path()
 {
     const char* e = std::getenv("AFX_TUNE_CACHE");
     if (e && *e) return std::strcmp(e, "off") == 0 ? std::string() : std::string(e);
-    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) return std::string(x) + "/afx/tune-v1.txt";
-    if (const char* h = std::getenv("HOME"); h && *h) return std::string(h) + "/.cache/afx/tune-v1.txt";
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) return std::string(x) + "/afx/tune-v2.txt";
+    if (const char* h = std::getenv("HOME"); h && *h) return std::string(h) + "/.cache/afx/tune-v2.txt";
     return std::string();
 }
 
@@ -2359,10 +2379,10 @@ void tune_file_load(int device)
             k.dims.push_back(v);
             q += n;
         }
-        int tile, tblock, rotate;
-        if (!ok || std::sscanf(q, "%d %d %d", &tile, &tblock, &rotate) != 3) continue;
-        if (tile < 0 || tile >= kNumTiles || (tblock != 1 && tblock != 2)) continue;
-        g_tune_cache[k] = TuneChoice{tile, tblock, rotate != 0};
+        int tile, tblock, rotate, split;
+        if (!ok || std::sscanf(q, "%d %d %d %d", &tile, &tblock, &rotate, &split) != 4) continue;
+        if (tile < 0 || tile >= kNumTiles || (tblock != 1 && tblock != 2) || split < 0 || split > 2) continue;
+        g_tune_cache[k] = TuneChoice{tile, tblock, rotate != 0, split};
     }
     std::fclose(f);
 }
@@ -2376,8 +2396,8 @@ void tune_file_store(const TuneKey& k, const TuneChoice& c)
         if (path[i] == '/') (void)mkdir(path.substr(0, i).c_str(), 0755);
     FILE* f = std::fopen(path.c_str(), "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d\n", tune_line_key(k, device_tag(k.device)).c_str(), c.tile, c.tblock,
-                 c.rotate ? 1 : 0);
+    std::fprintf(f, "%s %d %d %d %d\n", tune_line_key(k, device_tag(k.device)).c_str(), c.tile, c.tblock,
+                 c.rotate ? 1 : 0, c.split);
     std::fclose(f);
 }
 
@@ -2412,7 +2432,28 @@ extern "C" int afx_set_time_block(afx_sim* s, int32_t steps)
     }
     HIP_TRY(hipSetDevice(s->cfg.device));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    return set_tile_tb(s, s->tile, steps, s->xcd_rotate);
+    return set_tile_tb(s, s->tile, steps, s->xcd_rotate, s->split);
+}
+
+extern "C" int afx_set_tile_split(afx_sim* s, int32_t split)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (split < 0 || split > 2) return fail(AFX_E_ARG, "tile split must be 0, 1 or 2, not %d", split);
+    if (!s->committed) {
+        s->split = split;
+        return AFX_OK;
+    }
+    HIP_TRY(hipSetDevice(s->cfg.device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return set_tile_tb(s, s->tile, s->tblock, s->xcd_rotate, split);
+}
+
+extern "C" int afx_get_tile_split(afx_sim* s, int32_t* split)
+{
+    if (int rc = check_sim(s)) return rc;
+    if (!split) return fail(AFX_E_ARG, "null argument");
+    *split = s->split;
+    return AFX_OK;
 }
 
 extern "C" int afx_get_time_block(afx_sim* s, int32_t* steps, int64_t* pair_tiles)
@@ -2452,10 +2493,10 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
             choice = hit->second;
         }
     }
-    const TuneChoice current{s->tile, s->tblock, s->xcd_rotate};
+    const TuneChoice current{s->tile, s->tblock, s->xcd_rotate, s->split};
     if (cached) {
         if (choice != current)
-            if (int rc = set_tile_tb(s, choice.tile, choice.tblock, choice.rotate)) return rc;
+            if (int rc = set_tile_tb(s, choice.tile, choice.tblock, choice.rotate, choice.split)) return rc;
     } else {
         Snapshot sn;
         if (int rc = snap_take(s, sn)) return rc;
@@ -2467,7 +2508,7 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         for (int tl = 0; tl < kNumTiles; ++tl)
             for (int tb = 1; tb <= (can_pair(s, tl) ? 2 : 1) && tile_fits(s, tl); ++tb)
                 for (int ro = 1; ro >= 0; --ro) {
-                    const TuneChoice c{tl, tb, ro != 0};
+                    const TuneChoice c{tl, tb, ro != 0, 0};
                     if (c != current) cands.push_back(c);
                 }
         const size_t fbytes = s->P[0].bytes;
@@ -2477,10 +2518,11 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        for (size_t ci = 0; ci < cands.size(); ++ci) {
-            const bool is_ref = ci == 0;
+        // times one candidate (its best of three passes) and keeps it if it is
+        // bit-exact against the reference (the first candidate) and faster
+        auto trial = [&](const TuneChoice& cand, bool is_ref) -> int {
             if (int rc = snap_restore(s, sn)) return rc;
-            if (int rc = set_tile_tb(s, cands[ci].tile, cands[ci].tblock, cands[ci].rotate)) return rc;
+            if (int rc = set_tile_tb(s, cand.tile, cand.tblock, cand.rotate, cand.split)) return rc;
             // untimed pass (captures graphs, warms caches), then three timed
             // passes from the same state; the first is also the check pass
             if (int rc = run_steps(s, steps_per_trial)) return rc;
@@ -2498,16 +2540,26 @@ extern "C" int afx_autotune(afx_sim* s, int32_t steps_per_trial, int32_t tile[3]
                     HIP_TRY(hipMemcpy(is_ref ? ref.data() : got.data(), s->P[s->rot[0]].p, fbytes,
                                       hipMemcpyDeviceToHost));
             }
-            if (!is_ref && std::memcmp(ref.data(), got.data(), fbytes) != 0) continue;  // not bit-exact: rejected
+            if (!is_ref && std::memcmp(ref.data(), got.data(), fbytes) != 0) return AFX_OK;  // not bit-exact
             if (ms < best_ms) {
                 best_ms = ms;
-                best = cands[ci];
+                best = cand;
             }
+            return AFX_OK;
+        };
+        for (size_t ci = 0; ci < cands.size(); ++ci)
+            if (int rc = trial(cands[ci], ci == 0)) return rc;
+        // then the best layout with the top quarter and the top half of each
+        // room's tiles dealt out in halves
+        if (use_fused(s) && kTiles[best.tile].tz >= 2) {
+            const TuneChoice base = best;
+            for (int sp = 1; sp <= 2; ++sp)
+                if (int rc = trial(TuneChoice{base.tile, base.tblock, base.rotate, sp}, false)) return rc;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (int rc = snap_restore(s, sn)) return rc;
-        if (int rc = set_tile_tb(s, best.tile, best.tblock, best.rotate)) return rc;
+        if (int rc = set_tile_tb(s, best.tile, best.tblock, best.rotate, best.split)) return rc;
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         g_tune_cache[key] = best;
         tune_file_store(key, best);

@@ -118,11 +118,11 @@ __device__ __forceinline__ C phi_update(C old, C p1j, C p1n, C pj, C pn, C psj, 
 template <typename S, typename C, int TZ>
 __device__ __forceinline__ void interior_column(S* __restrict__ out, const S* __restrict__ cur,
                                                 const S* __restrict__ prev, const RoomDev& r, int x, int y,
-                                                int z0, C l2, C c2, uint32_t skipbits)
+                                                int z0, C l2, C c2, uint32_t skipbits, int zlo = 0, int zhi = TZ)
 {
     if (x < 1 || x >= r.nx - 1 || y < 1 || y >= r.ny - 1) return;
-    const int zs = z0 < 1 ? 1 : z0;
-    const int ze = (z0 + TZ) < (r.nz - 1) ? (z0 + TZ) : (r.nz - 1);
+    const int zs = z0 + zlo < 1 ? 1 : z0 + zlo;
+    const int ze = (z0 + zhi) < (r.nz - 1) ? (z0 + zhi) : (r.nz - 1);
     if (zs >= ze) return;
     const int64_t sy = r.nx, sz = (int64_t)r.nx * r.ny;
     int64_t i = r.base + (int64_t)zs * sz + (int64_t)y * sy + x;
@@ -198,11 +198,11 @@ __device__ __forceinline__ void ldv(C (&o)[V], const S* __restrict__ p, int64_t 
 template <typename S, typename C, int V, int TZ>
 __device__ __forceinline__ void interior_vec(S* __restrict__ out, const S* __restrict__ cur,
                                              const S* __restrict__ prev, const RoomDev& r, int x, int y, int z0,
-                                             C l2, C c2, const uint32_t (&skip)[V])
+                                             C l2, C c2, const uint32_t (&skip)[V], int zlo = 0, int zhi = TZ)
 {
     if (y < 1 || y >= r.ny - 1) return;  // the whole wave: one row
-    const int zs = z0 < 1 ? 1 : z0;
-    const int ze = (z0 + TZ) < (r.nz - 1) ? (z0 + TZ) : (r.nz - 1);
+    const int zs = z0 + zlo < 1 ? 1 : z0 + zlo;
+    const int ze = (z0 + zhi) < (r.nz - 1) ? (z0 + zhi) : (r.nz - 1);
     if (zs >= ze) return;
     const int lane = (int)(threadIdx.x & 63u);
     const bool inrow = x < r.nx;
@@ -286,17 +286,20 @@ __device__ __forceinline__ void interior_vec(S* __restrict__ out, const S* __res
 
 // The interior cells of one tile (origin ox, oy, oz of room r), one column
 // per thread or V columns per lane; cmask: the tile slot's column masks
-// (TX x TY words, bit z - oz set for cells left to a list), or null.
+// (TX x TY words, bit z - oz set for cells left to a list), or null.  Only
+// planes oz + zlo .. oz + zhi - 1 are updated (a tile may be dealt out in
+// parts).
 template <typename S, typename C, int TX, int TY, int TZ, int V>
 __device__ __forceinline__ void tile_interior(S* __restrict__ out, const S* __restrict__ cur,
                                               const S* __restrict__ prev, const RoomDev& r, int ox, int oy, int oz,
-                                              C l2, C c2, const uint32_t* __restrict__ cmask)
+                                              C l2, C c2, const uint32_t* __restrict__ cmask, int zlo = 0,
+                                              int zhi = TZ)
 {
     if constexpr (V == 1) {
         const int x = ox + (int)(threadIdx.x % TX);
         const int y = oy + (int)(threadIdx.x / TX);
         const uint32_t bits = cmask ? cmask[threadIdx.x] : 0u;
-        interior_column<S, C, TZ>(out, cur, prev, r, x, y, oz, l2, c2, bits);
+        interior_column<S, C, TZ>(out, cur, prev, r, x, y, oz, l2, c2, bits, zlo, zhi);
     } else {
         static_assert(TX == 64 * V, "one 64-lane wave per tile row");
         const int lane = (int)(threadIdx.x & 63u), row = (int)(threadIdx.x >> 6);
@@ -310,7 +313,7 @@ __device__ __forceinline__ void tile_interior(S* __restrict__ out, const S* __re
 #pragma unroll
             for (int k = 0; k < V; ++k) bits[k] = 0u;
         }
-        interior_vec<S, C, V, TZ>(out, cur, prev, r, ox + lane * V, oy + row, oz, l2, c2, bits);
+        interior_vec<S, C, V, TZ>(out, cur, prev, r, ox + lane * V, oy + row, oz, l2, c2, bits, zlo, zhi);
     }
 }
 
@@ -1260,10 +1263,14 @@ void k_fused(const int4* __restrict__ work, const int64_t* __restrict__ d_t, int
         }
     }
     if (wk.x == 0) {
-        const RoomDev r = wk.w == 0 ? c.room0 : c.rooms[wk.w];
+        // w: room (bits 0-15), then the tile's first and end plane of this
+        // item (bits 16-23, 24-31; an end of 0 is the whole tile)
+        const int room = wk.w & 0xFFFF, zlo = (wk.w >> 16) & 0xFF, zend = (int)((uint32_t)wk.w >> 24);
+        const RoomDev r = room == 0 ? c.room0 : c.rooms[room];
         const uint32_t o = (uint32_t)wk.z;
         tile_interior<S, C, TX, TY, TZ, V>(c.out, c.cur, c.prev, r, (int)(o & 2047u), (int)((o >> 11) & 2047u),
-                                           (int)(o >> 22), c.k.l2, c.k.c2, colmask + (int64_t)wk.y * NCOL);
+                                           (int)(o >> 22), c.k.l2, c.k.c2, colmask + (int64_t)wk.y * NCOL, zlo,
+                                           zend ? zend : TZ);
         return;
     }
     if (wk.x < 0) return;

@@ -331,6 +331,7 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
             elif base in ("fp16", "bf16", "fp32"):
                 t0 = time.perf_counter()
                 tuned = list(s.autotune(8)) + [s.time_block()[0]]
+                split = s.tile_split()
                 tune_s = time.perf_counter() - t0
             s.run(1)
             s.synchronize()
@@ -345,6 +346,7 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
         res[prec] = {"gcell_per_s": round(w.cells * (steps - 1) / dt / 1e9, 1), "wall_s": round(dt, 2)}
         if tuned is not None:
             res[prec]["autotuned_tile_and_steps_per_pass"] = tuned
+            res[prec]["tile_split"] = split
             res[prec]["tune_s"] = round(tune_s, 1)
         progress(f"large: {prec} done")
         if two:
@@ -539,6 +541,7 @@ def main():
                    "mode": "compat" if args.compat else "physical",
                    "boundary_cells": int(st.boundary), "fused_step_kernel": bool(st.fused),
                    "tile": [st.tile_x, st.tile_y, st.tile_z], "steps_per_pass": sim.time_block()[0],
+                   "tile_split": sim.tile_split(),
                    "autotuned": tile is not None,
                    "profile": {"step_ms": round(prof.step_ms, 5),
                                "kernel_ms": [round(x, 5) for x in prof.kernel_ms],

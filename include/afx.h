@@ -21,6 +21,7 @@
  *   afx_get_stats                        (boundary counts logged at lrs-data.hpp:212-214)
  *   afx_autotune                         (no counterpart: tile-shape and time-block search)
  *   afx_set_time_block, afx_get_time_block (no counterpart: two steps per pass)
+ *   afx_set_tile_split, afx_get_tile_split (no counterpart: tiles dealt out in halves)
  *   afx_profile_run                      (no counterpart: per-kernel event timing)
  *   afx_synchronize                      clh Device::queue().finish(), simulation.hpp:407
  *
@@ -202,16 +203,26 @@ int afx_set_time_block(afx_sim* sim, int32_t steps);
 /* The steps per pass set, and (pair_tiles, may be NULL) how many tiles take
  * two steps per pass on the committed handle (0 when single steps run). */
 int afx_get_time_block(afx_sim* sim, int32_t* steps, int64_t* pair_tiles);
+/* The fused step deals the tiles whose origin lies in the top quarter
+ * (split 1) or the top half (split 2) of each room's planes as two work items
+ * of half the tile's planes each; 0 (the default) keeps whole tiles.  The
+ * step's last workgroups are then shorter, and so is the time the step waits
+ * for them.  Results are bit-identical for every setting; afx_autotune
+ * chooses it along with the tile shape. */
+int afx_set_tile_split(afx_sim* sim, int32_t split);
+int afx_get_tile_split(afx_sim* sim, int32_t* split);
 
 /* Times the candidate interior tile shapes, each with one and two steps per
  * pass where that applies and with two assignments of tile layers to the
- * XCDs, on this handle's rooms (each is first checked bit-exact against the
+ * XCDs, then the fastest of them with tiles split (afx_set_tile_split 1 and
+ * 2), on this handle's rooms (each is first checked bit-exact against the
  * current setting on a copy of the state; the best of three timed passes
  * counts), keeps the fastest and returns its tile in tile[3]
- * (afx_get_time_block gives its steps per pass).  Leaves the simulation
- * state unchanged.  Results are kept per (device, precision, mode, PML,
- * flags, room dims) for the process and in the file AFX_TUNE_CACHE (default
- * $XDG_CACHE_HOME/afx/tune-v1.txt or $HOME/.cache/afx/tune-v1.txt;
+ * (afx_get_time_block gives its steps per pass, afx_get_tile_split its
+ * split).  Leaves the simulation state unchanged.  Results are kept per
+ * (device, precision, mode, PML, flags, room dims) for the process and in the
+ * file AFX_TUNE_CACHE (default $XDG_CACHE_HOME/afx/tune-v2.txt or
+ * $HOME/.cache/afx/tune-v2.txt;
  * "off" disables it), so a later process with the same key skips the
  * search. */
 int afx_autotune(afx_sim* sim, int32_t steps_per_trial, int32_t tile[3]);

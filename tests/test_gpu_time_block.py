@@ -167,6 +167,49 @@ def test_time_block_argument_checks():
     s.close()
 
 
+@pytest.mark.parametrize("tile,tblock,split", [((64, 4, 16), 1, 1), ((64, 4, 16), 1, 2), ((64, 4, 16), 2, 2),
+                                               ((128, 4, 8), 1, 2), ((64, 2, 32), 2, 1)])
+def test_tile_split_matches_oracle(tile, tblock, split):
+    """Tiles in the top quarter (split 1) or half (split 2) of the room's
+    planes run as two work items of half the planes each; with one or two
+    steps per pass and vector tiles the result stays the oracle's, bit for
+    bit, including a switch of the split mid-run."""
+    _case(f"tile split {split}, tile {tile}, steps per pass {tblock}, 60 + 61 steps")
+    v, a, b, sc, mc, prof = _room()
+    steps = 121
+    src = _src(steps)
+    s = _sim(v, a, b, sc, mc, prof, src, tile=tile, tblock=tblock)
+    s.set_tile_split(split)
+    s.run(60)
+    assert s.tile_split() == split
+    s.set_tile_split(0)
+    s.run(30)
+    s.set_tile_split(split)
+    s.run(31)
+    got, f = s.receivers(), s.pressure(0)
+    s.close()
+    dt = rooms.courant_dt(0.02)
+    o = orc.Oracle(v, a, b, rooms.C_AIR, 0.02, dt, False, sc, src, mc, "f32", 2, 6, 2, prof)
+    o.run(steps)
+    _assert_bitexact(got, o.receivers(), "traces")
+    _assert_bitexact(f, o.field(0), "field")
+    assert np.abs(got).max() > 0
+
+
+def test_tile_split_argument_checks():
+    _case("tile split argument checks")
+    v, a, b, sc, mc, prof = _room()
+    s = _sim(v, a, b, sc, mc, prof, _src(4))
+    assert s.tile_split() == 0
+    for bad in (-1, 3):
+        with pytest.raises(afx.AfxError):
+            s.set_tile_split(bad)
+    s.set_tile_split(2)
+    s.run(4)
+    assert s.tile_split() == 2
+    s.close()
+
+
 def test_profiled_steps_with_two_steps_per_pass():
     """afx_profile_run (events around every launch, no graphs) with two steps
     per pass: 33 steps (16 passes and one single step) continue the state

@@ -18,7 +18,7 @@ s = rooms.make_sim(w, _afx.Sim)
 t0 = time.perf_counter()
 tile = s.autotune(10)
 dt = time.perf_counter() - t0
-print(*tile, s.time_block()[0], f"{dt:.4f}")
+print(*tile, s.time_block()[0], s.tile_split(), f"{dt:.4f}")
 s.close()
 """
 
@@ -28,7 +28,7 @@ def _child(env):
                          capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     f = out.stdout.split()
-    return tuple(int(v) for v in f[:4]), float(f[4])
+    return tuple(int(v) for v in f[:5]), float(f[5])
 
 
 def test_autotune_result_survives_the_process(tmp_path):
@@ -47,9 +47,10 @@ def test_autotune_result_survives_the_process(tmp_path):
     print(f"search {t_first:.3f} s, cached {t_second:.4f} s")
     assert t_second < t_first
     # lines that do not parse or name an unknown tile are ignored: the third
-    # process searches again and appends its result
-    bad = " ".join(fields[:-3] + ["99"] + fields[-2:])
+    # process searches again and appends its result (key, then tile, steps
+    # per pass, XCD rotation and split)
+    bad = " ".join(fields[:-4] + ["99"] + fields[-3:])
     path.write_text("garbage line\n" + bad + "\n")
     _child(env)
     after = path.read_text().splitlines()
-    assert len(after) == 3 and after[2].split()[:-3] == fields[:-3]
+    assert len(after) == 3 and after[2].split()[:-4] == fields[:-4]
