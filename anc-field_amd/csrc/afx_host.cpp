@@ -871,9 +871,25 @@ int build_special(afx_sim* s)
         }
         HIP_TRY(upload(s->d_e_pack, epack, s->stream));
         HIP_TRY(upload(s->d_pml_nbd, nbd, s->stream));
-        auto interleave = [](std::vector<std::vector<int4>> sq) {
+        auto interleave = [s](std::vector<std::vector<int4>> sq) {
             balance_kind(sq, 1);
             balance_kind(sq, 2);
+            if (s->split > 0) {
+                // From the first tile half of a room's top quarter on, an XCD's
+                // list chunks go before its tile halves, so the sequence ends
+                // with the shortest workgroups.
+                for (auto& v : sq) {
+                    size_t p0 = v.size();
+                    for (size_t k = 0; k < v.size() && p0 == v.size(); ++k) {
+                        const int4& w = v[k];
+                        if (w.x != 0 || ((uint32_t)w.w >> 24) == 0) continue;
+                        const Room& rr = s->rooms[(size_t)(w.w & 0xFFFF)];
+                        if (4 * (int)((uint32_t)w.z >> 22) >= 3 * rr.nz) p0 = k;
+                    }
+                    std::stable_partition(v.begin() + (std::ptrdiff_t)p0, v.end(),
+                                          [](const int4& w) { return w.x != 0 && w.x != 4; });
+                }
+            }
             size_t len = 0;
             for (const auto& v : sq) len = std::max(len, v.size());
             std::vector<int4> work(std::max<size_t>(8 * len, 1), make_int4(-1, 0, 0, 0));
