@@ -875,15 +875,18 @@ int build_special(afx_sim* s)
             balance_kind(sq, 1);
             balance_kind(sq, 2);
             if (s->split > 0) {
-                // From the first tile half of a room's top quarter on, an XCD's
-                // list chunks go before its tile halves, so the sequence ends
-                // with the shortest workgroups.
+                // From the first tile half of the top quarter of an XCD's last
+                // room on, its list chunks go before its tile halves, so the
+                // sequence ends with the shortest workgroups.
                 for (auto& v : sq) {
+                    int last_room = -1;
+                    for (size_t k = v.size(); k-- > 0 && last_room < 0;)
+                        if (v[k].x == 0 || v[k].x == 4) last_room = v[k].w & 0xFFFF;
                     size_t p0 = v.size();
-                    for (size_t k = 0; k < v.size() && p0 == v.size(); ++k) {
+                    for (size_t k = 0; k < v.size() && p0 == v.size() && last_room >= 0; ++k) {
                         const int4& w = v[k];
-                        if (w.x != 0 || ((uint32_t)w.w >> 24) == 0) continue;
-                        const Room& rr = s->rooms[(size_t)(w.w & 0xFFFF)];
+                        if (w.x != 0 || ((uint32_t)w.w >> 24) == 0 || (w.w & 0xFFFF) != last_room) continue;
+                        const Room& rr = s->rooms[(size_t)last_room];
                         if (4 * (int)((uint32_t)w.z >> 22) >= 3 * rr.nz) p0 = k;
                     }
                     std::stable_partition(v.begin() + (std::ptrdiff_t)p0, v.end(),
