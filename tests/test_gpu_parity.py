@@ -224,13 +224,17 @@ def test_batched_rooms_match_single():
         k += len(w.mic_cells)
 
 
-@pytest.mark.parametrize("precision,tblock,compat", [("fp32", 1, False), ("fp32", 2, False), ("fp16", 1, False),
-                                                    ("fp32", 2, True), ("fp16", 1, True)])
-def test_batched_rooms_match_oracle(precision, tblock, compat):
+@pytest.mark.parametrize("precision,tblock,compat,split", [("fp32", 1, False, 0), ("fp32", 2, False, 0),
+                                                          ("fp16", 1, False, 0), ("fp32", 2, True, 0),
+                                                          ("fp16", 1, True, 0), ("fp32", 1, False, 2),
+                                                          ("fp32", 2, True, 1)])
+def test_batched_rooms_match_oracle(precision, tblock, compat, split):
     """Each room of one handle against its own oracle run, bit for bit:
     physical mode with a PML on x+ and z- of every room, or compat mode (the
-    reference's constants; its PML is single-room only, so none)."""
-    _case(f"batched rooms vs the oracle per room, {precision}, {tblock} step(s) per pass, compat {compat}")
+    reference's constants; its PML is single-room only, so none); whole tiles
+    or the rooms' top tiles split in halves."""
+    _case(f"batched rooms vs the oracle per room, {precision}, {tblock} step(s) per pass, compat {compat}, "
+          f"split {split}")
     ws = rooms.batch_rooms(seed=11, nrooms=3, steps=130, lo=(70, 24, 24), hi=(140, 40, 40), nmic=3)
     faces = afx.FACE_XP | afx.FACE_ZM
     width = 5
@@ -242,6 +246,7 @@ def test_batched_rooms_match_oracle(precision, tblock, compat):
     if not compat:
         s.set_pml(width, faces, prof)
     s.set_time_block(tblock)
+    s.set_tile_split(split)
     ids = [0] + [s.add_room(w.vox, w.a, w.b) for w in ws[1:]]
     for rid, w in zip(ids, ws):
         for cell, sig in zip(w.src_cells, w.src):
