@@ -318,7 +318,8 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
     noise, 32 receivers.  The same run in FP64, FP32, FP16 and BF16 storage;
     receiver-trace error of each against FP64.  FP32 and FP64 once more with
     two steps per pass (time-step blocking), whose traces must equal the
-    single-step ones bit for bit."""
+    single-step ones bit for bit.  A name ending in `one_step_per_pass` runs
+    the default tile with one step per pass and no autotuning."""
     import numpy as np
     from anc_field_py import _afx, rooms
     w = rooms.large(steps=steps)
@@ -326,12 +327,13 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
     traces = {}
     for prec in precisions:
         two = prec.endswith("two_steps_per_pass")
+        one = prec.endswith("one_step_per_pass")
         base = prec.split("_")[0]
         s = rooms.make_sim(w, _afx.Sim, precision=base)
         try:
             tuned = None
-            if two:
-                s.set_time_block(2)
+            if two or one:
+                s.set_time_block(2 if two else 1)
             elif base in ("fp16", "bf16", "fp32"):
                 t0 = time.perf_counter()
                 tuned = list(s.autotune(8)) + [s.time_block()[0]]
@@ -355,6 +357,7 @@ def large_leg(steps, precisions=("fp64", "fp32", "fp16", "bf16", "fp32_two_steps
         progress(f"large: {prec} done")
         if two:
             res[prec]["pair_tiles"] = int(pairs)
+        if two or one:
             if base in traces:
                 res[prec][f"bitexact_vs_{base}"] = bool(np.array_equal(traces[prec], traces[base]))
     if "fp64" in traces:
