@@ -186,7 +186,9 @@ int afx_profile_run(afx_sim* sim, int64_t steps, afx_profile* out);
 
 /* Selects one of the compiled interior tile shapes (x, y, z cells per
  * workgroup): 64x4x16 (default), 64x4x32, 64x8x16, 64x2x32, 128x2x16, 64x4x8,
- * 256x1x16.  Results do not depend on the shape. */
+ * 256x1x16, and with 4 cells of x per lane 256x4x8 and 256x2x8, with 2 cells
+ * 128x4x16 and 128x4x8 (these need every room width to be a multiple of that
+ * count).  Results do not depend on the shape. */
 int afx_set_tile(afx_sim* sim, int32_t tx, int32_t ty, int32_t tz);
 
 /* Time-step blocking.  With steps = 2 each pass over the grid advances two
