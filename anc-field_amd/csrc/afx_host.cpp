@@ -524,6 +524,50 @@ std::vector<std::array<int, 7>> pml_box_list(const afx_sim* s)
 // computes the step value, adds the sources in order and records receivers.
 bool use_fused(const afx_sim* s);
 
+// Physical PML: graded zeta on the selected faces of every room, at cell
+// centres (zc) and +1/2 points (zh), in double; the per-axis profiles of the
+// rooms are concatenated, dims + 1 entries each (RoomDev::axis_off).
+void pml_zeta_tables(const afx_sim* s, std::vector<double> zc[3], std::vector<double> zh[3])
+{
+    const int w = s->pml_width;
+    const double order = s->cfg.pml_order > 0 ? s->cfg.pml_order : 3.0;
+    const double r0 = s->cfg.pml_r0 > 0 ? s->cfg.pml_r0 : 1e-3;
+    const double zmax = (order + 1.0) * s->cfg.c * std::log(1.0 / r0) / (2.0 * w * s->cfg.dx);
+    auto prof = [&](double depth) -> double {  // depth in cells from the interface
+        if (depth <= 0) return 0.0;
+        if (!s->pml_profile.empty()) {
+            // user values at cell-centre depths 0.5 .. w-0.5, linear in between
+            const double u = depth - 0.5;
+            const int lo = (int)std::floor(u);
+            const double f = u - lo;
+            auto at = [&](int k) { return k < 0 ? 0.0 : (k >= w ? (double)s->pml_profile[(size_t)w - 1] : (double)s->pml_profile[(size_t)k]); };
+            return (1.0 - f) * at(lo) + f * at(lo + 1);
+        }
+        return zmax * std::pow(std::min(depth, (double)w) / w, order);
+    };
+    for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+        const Room& rm = s->rooms[ri];
+        const int rdims[3] = {rm.nx, rm.ny, rm.nz};
+        for (int d = 0; d < 3; ++d) {
+            const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
+            const int n = rdims[d];
+            for (int i = 0; i <= n; ++i) {
+                double vc = 0, vh = 0;
+                if (i < n && hi) {  // interface at n-1-w-0.5
+                    vc = std::max(vc, prof(i - (n - 1 - w) + 0.5));
+                    vh = std::max(vh, prof(i - (n - 1 - w) + 1.0));
+                }
+                if (i < n && lo) {  // interface at w+0.5
+                    vc = std::max(vc, prof((w + 0.5) - i));
+                    vh = std::max(vh, prof((double)(w - i)));
+                }
+                zc[d].push_back(vc);
+                zh[d].push_back(vh);
+            }
+        }
+    }
+}
+
 int build_special(afx_sim* s)
 {
     struct Sp {
@@ -925,6 +969,20 @@ int build_special(afx_sim* s)
                                         (boxes[b][6] - boxes[b][5]);
         std::vector<int4> rec(pcell.size());
         std::vector<uint32_t> side(pcell.size());
+        // zeta at cell centres as the kernels hold it (FP64, else FP32), and
+        // each room's offset in the concatenated per-axis tables
+        std::vector<double> zc[3], zh[3];
+        if (s->pml_mode == 2) pml_zeta_tables(s, zc, zh);
+        std::vector<std::array<int64_t, 3>> zoff(s->rooms.size() + 1, {0, 0, 0});
+        for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
+            const int rd[3] = {s->rooms[ri].nx, s->rooms[ri].ny, s->rooms[ri].nz};
+            for (int d = 0; d < 3; ++d) zoff[ri + 1][d] = zoff[ri][d] + rd[d] + 1;
+        }
+        auto zeta_nonzero = [&](int room, int d, int i) {
+            if (zc[d].empty()) return true;
+            const double v = zc[d][(size_t)(zoff[(size_t)room][d] + i)];
+            return s->prec == Prec::F64 ? v != 0.0 : (float)v != 0.0f;
+        };
         for (size_t k = 0; k < pcell.size(); ++k) {
             const int room = (int)((uint32_t)pflag[k] >> 24);
             const int x = (int)(pxyz[k] & 2047u), y = (int)((pxyz[k] >> 11) & 2047u), z = (int)(pxyz[k] >> 22);
@@ -952,6 +1010,8 @@ int build_special(afx_sim* s)
                 }
             }
             if (other) nb |= 1u << 28;
+            if ((int)zeta_nonzero(room, 0, x) + (int)zeta_nonzero(room, 1, y) + (int)zeta_nonzero(room, 2, z) >= 2)
+                nb |= 1u << 29;
             rec[k] = make_int4((int)pxyz[k], pflag[k], (int)kc, (int)nb);
             side[k] = (uint32_t)box | (other << 26);
         }
@@ -1050,45 +1110,12 @@ int setup_pml_t(afx_sim* s)
         s->pml_rot = 0;
         return AFX_OK;
     }
-    // physical: graded zeta on the selected faces of every room; the per-axis
-    // profiles of the rooms are concatenated (RoomDev::axis_off)
-    const int w = s->pml_width;
-    const double order = s->cfg.pml_order > 0 ? s->cfg.pml_order : 3.0;
-    const double r0 = s->cfg.pml_r0 > 0 ? s->cfg.pml_r0 : 1e-3;
-    const double zmax = (order + 1.0) * s->cfg.c * std::log(1.0 / r0) / (2.0 * w * s->cfg.dx);
-    auto prof = [&](double depth) -> double {  // depth in cells from the interface
-        if (depth <= 0) return 0.0;
-        if (!s->pml_profile.empty()) {
-            // user values at cell-centre depths 0.5 .. w-0.5, linear in between
-            const double u = depth - 0.5;
-            const int lo = (int)std::floor(u);
-            const double f = u - lo;
-            auto at = [&](int k) { return k < 0 ? 0.0 : (k >= w ? (double)s->pml_profile[(size_t)w - 1] : (double)s->pml_profile[(size_t)k]); };
-            return (1.0 - f) * at(lo) + f * at(lo + 1);
-        }
-        return zmax * std::pow(std::min(depth, (double)w) / w, order);
-    };
+    std::vector<double> zc_d[3], zh_d[3];
+    pml_zeta_tables(s, zc_d, zh_d);
     std::vector<C> zc_all[3], zh_all[3];
-    for (size_t ri = 0; ri < s->rooms.size(); ++ri) {
-        const Room& rm = s->rooms[ri];
-        const int rdims[3] = {rm.nx, rm.ny, rm.nz};
-        for (int d = 0; d < 3; ++d) {
-            const bool lo = (s->pml_faces >> (2 * d)) & 1u, hi = (s->pml_faces >> (2 * d + 1)) & 1u;
-            const int n = rdims[d];
-            for (int i = 0; i <= n; ++i) {
-                double vc = 0, vh = 0;
-                if (i < n && hi) {  // interface at n-1-w-0.5
-                    vc = std::max(vc, prof(i - (n - 1 - w) + 0.5));
-                    vh = std::max(vh, prof(i - (n - 1 - w) + 1.0));
-                }
-                if (i < n && lo) {  // interface at w+0.5
-                    vc = std::max(vc, prof((w + 0.5) - i));
-                    vh = std::max(vh, prof((double)(w - i)));
-                }
-                zc_all[d].push_back((C)vc);
-                zh_all[d].push_back((C)vh);
-            }
-        }
+    for (int d = 0; d < 3; ++d) {
+        zc_all[d].assign(zc_d[d].begin(), zc_d[d].end());
+        zh_all[d].assign(zh_d[d].begin(), zh_d[d].end());
     }
     for (int d = 0; d < 3; ++d) {
         HIP_TRY(upload(s->d_z[d], zc_all[d], s->stream));

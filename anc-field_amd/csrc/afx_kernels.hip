@@ -669,7 +669,9 @@ struct BndCtx {
     // 0-10 / 11-21 the x / y extent of its box, bits 22-27 which of the
     // neighbours +x, -x, +y, -y, +z, -z lie in the same box, bit 28 set when
     // one of the others lies in another PML box: then pml_side[q] holds the
-    // box (bits 0-25) and those neighbours (bits 26-31), found by pml_find
+    // box (bits 0-25) and those neighbours (bits 26-31), found by pml_find;
+    // bit 29 set when at least two of the cell's zeta are non-zero (only
+    // then does the fused PML chunk read and store the cell's psi)
     const int4* pml_rec;
     const uint32_t* pml_side;
     const int32_t* pml_boxes;
@@ -952,7 +954,14 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
         C zeta[3], hp[3], hm[3], oldp[3], oldm[3], np1[3], nm1[3], npv[3], nmv[3], psp[3], psm[3];
         pc = ld<S, C>(c.cur, i);
         pm = ld<S, C>(c.prev, i);
-        ps = psr[k];
+        // psi enters the update only through products with zeta_a * zeta_b
+        // (a != b) taken at this cell's coordinates, so where fewer than two
+        // of the cell's zeta are non-zero (record bit 29 clear: a cell of a
+        // one-face slab) those products are zero whatever psi holds, and psi
+        // is neither read nor stored.
+        const bool live = (u.nb >> 29) & 1u;
+        ps = 0;
+        if (live) ps = psr[k];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const C* ph = c.phib[par ^ 1][d];
@@ -967,8 +976,12 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
             nm1[d] = ld<S, C>(c.cur, i - st[d]);
             npv[d] = ld<S, C>(c.prev, i + st[d]);
             nmv[d] = ld<S, C>(c.prev, i - st[d]);
-            psp[d] = psr[kpd];
-            psm[d] = psr[kmd];
+            psp[d] = 0;
+            psm[d] = 0;
+            if (live) {
+                psp[d] = psr[kpd];
+                psm[d] = psr[kmd];
+            }
         }
         // ---- phi^n at the cell (phi_at with kj = k, kn = kp[d]) and at its
         // - neighbours (kj = km[d], kn = k); g2, g3 as phi_at forms them
@@ -1005,7 +1018,7 @@ __device__ __forceinline__ void pml_chunk(const BndCtx<S, C>& c, int a, int b, i
             c.phib[par][0][k] = fx;
             c.phib[par][1][k] = fy;
             c.phib[par][2][k] = fz;
-            c.psib[par][k] = psn;
+            if (live) c.psib[par][k] = psn;
         }
         lds[0][tid] = fx;
         lds[1][tid] = fy;
