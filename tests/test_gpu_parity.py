@@ -164,6 +164,34 @@ def test_physical_pml_bitexact(fused, list_blocks, precision):
     _assert_bitexact(f, wf, "field")
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_physical_pml_three_face_corner(precision):
+    """PML on x+, y+ and z+: one-face slabs (one non-zero zeta: the fused
+    chunks keep no psi there), two-face edges and the three-face corner (psi
+    read and stored, and s3 = zeta_x zeta_y zeta_z non-zero), with a source in
+    the corner and receivers in each kind of region."""
+    _case(f"physical PML x+ y+ z+ corner {precision}")
+    v, a, b, rng = _closed_room(29, dims=(48, 40, 30))
+    v[:, :, -1] = -1   # open x+ face
+    v[:, -1, :] = -1   # open y+ face
+    v[-1, :, :] = -1   # open z+ face
+    v = rooms.repair(v)
+    inner = rooms.interior_cells(v, rng, 2)
+    corner = np.array([44, 36, 26], np.int32)    # all three zeta non-zero (width 6)
+    edge = np.array([44, 36, 12], np.int32)      # x+ and y+
+    slab = np.array([20, 20, 27], np.int32)      # z+ only
+    sc = np.stack([inner[0], corner])
+    mc = np.stack([inner[1], corner, edge, slab])
+    steps = 300
+    dt = rooms.courant_dt(0.02)
+    src = np.stack([rooms.ricker(steps, dt), rooms.ricker(steps, dt, 900.0, amp=0.5)])
+    prof = (3e4 * ((np.arange(6) + 0.5) / 6) ** 3).astype(np.float32)
+    got, want, f, wf = _run_pair(v, a, b, False, sc, src, mc, steps, precision=precision, pml=(6, 2 | 8 | 32, prof))
+    _assert_bitexact(got, want, "traces")
+    _assert_bitexact(f, wf, "field")
+    assert np.all(np.abs(np.asarray(got)).max(axis=1) > 0)  # the wave reached every receiver
+
+
 @pytest.mark.parametrize("profile", [None, "shipped"])
 def test_compat_pml_bitexact(profile):
     _case(f"compat PML profile={profile}")
